@@ -1,0 +1,208 @@
+/*
+ * umpalumpa_amd -- C ABI of the MI355X-native Fourier-reconstruction (FR) insertion and
+ * Fourier-processing (FP) crop operations.
+ *
+ * Every entry point names the reference interface it replaces (paths relative to the reference
+ * repository root).  Conventions shared by all entry points:
+ *   - status codes, never exceptions: 0 = OK, see UMPA_* below;
+ *   - the caller owns every buffer ("We do not manage your memory", data/payload.hpp:7-11);
+ *     data pointers passed to *_execute are device (or managed) pointers;
+ *   - *_execute enqueues on the operation's HIP stream and returns; *_synchronize blocks
+ *     (FRCUDA::Synchronize, fr_cuda.cpp:243).  Handles are not thread-safe individually;
+ *     distinct handles are independent.
+ */
+#ifndef UMPALUMPA_AMD_H
+#define UMPALUMPA_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (BasicOperation::Init/Execute return bool, basic_operation.hpp:28-66) --- */
+#define UMPA_OK 0
+#define UMPA_ERR_INVALID_ARG 1     /* malformed descriptor / pointer / size: Init would be false */
+#define UMPA_ERR_NOT_INITIALIZED 2 /* Execute before a successful Init (basic_operation.hpp:62) */
+#define UMPA_ERR_NOT_EQUIVALENT 3  /* payload not equivalent to the Init one (:62-63) */
+#define UMPA_ERR_REJECTED 4        /* the reference Execute returns false (fr_cpu.cpp:27-29) */
+#define UMPA_ERR_UNSUPPORTED 5     /* valid in the reference, not offered here */
+#define UMPA_ERR_DEVICE 6          /* HIP runtime error */
+#define UMPA_ERR_OUT_OF_MEMORY 7
+
+/* Size (data/size.hpp:23-103): x, y, z, n (n = number of objects) */
+typedef struct umpa_size {
+  uint64_t x, y, z, n;
+} umpa_size;
+
+/* TraverseSpace (fourier_reconstruction/traverse_space.hpp:14-48), byte compatible: 112 B */
+typedef struct umpa_point3 {
+  float x, y, z;
+} umpa_point3;
+typedef struct umpa_traverse_space {
+  int32_t minY, minX, minZ;
+  int32_t maxY, maxX, maxZ;
+  float maxDistanceSqr;
+  int32_t dir; /* 0 = XY, 1 = XZ, 2 = YZ */
+  umpa_point3 unitNormal;
+  umpa_point3 topOrigin, bottomOrigin;
+  int32_t projectionIndex;
+  float transformInv[3][3];
+  float weight;
+} umpa_traverse_space;
+
+/* Constants (fourier_reconstruction/constants.hpp:6-16), 32 B */
+typedef struct umpa_fr_constants {
+  int32_t cMaxVolumeIndexX;
+  int32_t cMaxVolumeIndexYZ;
+  float cBlobRadius;
+  float cOneOverBlobRadiusSqr;
+  float cBlobAlpha;
+  float cIw0;
+  float cIDeltaSqrt;
+  float cOneOverBessiOrderAlpha;
+} umpa_fr_constants;
+
+/* fourier_reconstruction::Settings (settings.hpp:8-67) */
+#define UMPA_FR_DYNAMIC 0 /* Interpolation::kDynamic */
+#define UMPA_FR_LOOKUP 1  /* Interpolation::kLookup */
+#define UMPA_FR_FAST 0    /* Type::kFast (nearest neighbour) */
+#define UMPA_FR_PRECISE 1 /* Type::kPrecise (blob) */
+typedef struct umpa_fr_settings {
+  int32_t interpolation;
+  int32_t order; /* BlobOrder k0..k4 (blob_order.hpp:5-23) */
+  int32_t type;
+  float alpha;       /* default 15 */
+  float blobRadius;  /* default 1.9 */
+} umpa_fr_settings;
+
+/* Shapes of AFR's payloads (afr.hpp:13-55, fr.cpp:240-249).
+ *   fft:    frequency-domain size of the slices, x = Hx, y = Hy, z = 1, n = slice count
+ *   volume: frequency-domain size of the volume (hasSymetry: full x stored), z = y, n = 1
+ *   traverseSpaces: capacity (Size(1,1,1,n)); blobTable: element count (0 for kDynamic) */
+typedef struct umpa_fr_desc {
+  umpa_size fft;
+  umpa_size volume;
+  uint64_t traverseSpaces;
+  uint64_t blobTable;
+} umpa_fr_desc;
+
+/* Precision of the slice data handed to execute */
+#define UMPA_SLICE_F32 0 /* interleaved complex float (float2), 8 B per pixel */
+#define UMPA_SLICE_F16 1 /* interleaved complex half + one float scale per slice, 4 B per pixel */
+
+/* Kernel strategy of an FR operation (the counterpart of FRCUDA's tuned kernel variants) */
+#define UMPA_FR_EXACT 0 /* FRCPU arithmetic and order: bit-identical to the reference CPU path */
+#define UMPA_FR_FAST_MATH 1 /* fused multiply-add, same order; within 1e-5 rel-L2 of FRCPU */
+
+typedef struct umpa_fr_op umpa_fr_op;
+
+/* ---- FR: the AFR operation, executed by hand-written gfx950 kernels (drop-in for FRCUDA,
+ *      fr_cuda.cpp:12-250, beside FRCPU, fr_cpu.cpp:9-54) ----------------------------------- */
+
+/* New operation bound to `device` and HIP stream `stream` (NULL = the null stream).
+ * Replaces constructing FRCUDA/FRCPU (fr_cuda.hpp, fr_cpu.hpp). */
+int umpa_fr_create(umpa_fr_op **op, int device, void *stream);
+
+/* AFR::Init (basic_operation.hpp:28-48 + Strategy1::Init, fr_cpu.cpp:14-19,
+ * fr_cuda.cpp:71-75): false (UMPA_ERR_INVALID_ARG) for an even volume x, invalid sizes or a
+ * volume whose y != z.  Allocates the operation's own brick schedule on the device. */
+int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_settings *settings);
+
+/* AFR::Execute (basic_operation.hpp:60-66 + FRCPU Strategy1::Execute, fr_cpu.cpp:21-44).
+ * Accumulates every traverse space into `volume` (complex float, [z][y][x]) and `weights`
+ * (float, [z][y][x]) in place; the caller zeroes them first (fr.cpp:46).  `fft` holds
+ * fftCount slices (n of the Init descriptor or fewer), `spaces` nSpaces records (device
+ * memory), `table` the blob table (device memory, blobTable elements, kLookup only).
+ * Returns UMPA_ERR_REJECTED where FRCPU returns false (kLookup without a table). */
+int umpa_fr_execute(umpa_fr_op *op, const void *fft, uint64_t fftCount, void *volume,
+                    float *weights, const umpa_traverse_space *spaces, uint64_t nSpaces,
+                    const float *table);
+
+/* Same as umpa_fr_execute with FP16 slices: `fft` is complex half [n][Hy][Hx] and `scales`
+ * one float per slice (pixel = half value * scale); accumulation stays FP32.  North-star
+ * precision mode; no counterpart in the reference. */
+int umpa_fr_execute_f16(umpa_fr_op *op, const void *fft, const float *scales,
+                        uint64_t fftCount, void *volume, float *weights,
+                        const umpa_traverse_space *spaces, uint64_t nSpaces,
+                        const float *table);
+
+/* Select the kernel strategy (UMPA_FR_EXACT default).  No counterpart in the reference API:
+ * FRCUDA picks kernel variants through KTT (fr_cuda.cpp:100-157). */
+int umpa_fr_set_strategy(umpa_fr_op *op, int strategy);
+
+/* FRCUDA::Synchronize (fr_cuda.cpp:243): wait for every enqueued execute */
+int umpa_fr_synchronize(umpa_fr_op *op);
+
+/* BasicOperation::Cleanup (basic_operation.hpp:72-79): drop Init state and device buffers */
+int umpa_fr_cleanup(umpa_fr_op *op);
+
+/* destroy the handle (the operation's destructor, basic_operation.hpp:14) */
+void umpa_fr_destroy(umpa_fr_op *op);
+
+/* voxel-updates counted by the last execute when counting is enabled (0 otherwise) */
+int umpa_fr_set_count_updates(umpa_fr_op *op, int enable);
+int umpa_fr_last_updates(umpa_fr_op *op, uint64_t *updates);
+
+/* ---- FR host-side helpers (no GPU needed) ----------------------------------------------- */
+
+/* AFR::CreateConstants (afr.cpp:6-19) */
+int umpa_fr_create_constants(const umpa_fr_desc *desc, const umpa_fr_settings *settings,
+                             umpa_fr_constants *out);
+
+/* AFR::FillBlobTable (afr.cpp:22-34): host memory, n = blobTable elements */
+int umpa_fr_fill_blob_table(float *table, uint64_t n, const umpa_fr_desc *desc,
+                            const umpa_fr_settings *settings);
+
+/* ComputeTraverseSpace (traverse_space_generator.hpp:94-145); transform is row-major 3x3 */
+int umpa_fr_compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY,
+                                   const float transform[9], umpa_traverse_space *space,
+                                   uint32_t maxVolumeIndexX, uint32_t maxVolumeIndexYZ,
+                                   int useFast, float blobRadius, float weight);
+
+/* ---- FP: the AFP operation (afp.hpp:13-53), crop/filter/normalize/center/shift ----------- */
+
+/* fourier_processing::Settings (fourier_processing/settings.hpp:10-84), out-of-place only */
+typedef struct umpa_fp_settings {
+  int32_t center;
+  int32_t normalize;
+  int32_t applyFilter;
+  int32_t shift;
+  int32_t hasMaxFreq;
+  float maxFreq;
+} umpa_fp_settings;
+
+/* Payload shapes: in/out are frequency-domain sizes (x = X/2+1), inSpatial/outSpatial the
+ * spatial sizes they came from; inPaddedSpatialSingle = elements of one padded spatial
+ * image (the normalization is 1/that, fp_cpu.cpp:42). */
+typedef struct umpa_fp_desc {
+  umpa_size in;
+  umpa_size inSpatial;
+  umpa_size out;
+  uint64_t inPaddedSpatialSingle;
+} umpa_fp_desc;
+
+typedef struct umpa_fp_op umpa_fp_op;
+
+/* new FP operation (FPCUDA / FPCPU) on `device` and `stream` */
+int umpa_fp_create(umpa_fp_op **op, int device, void *stream);
+/* AFP::Init (basic_operation.hpp:28-48, fp_cpu.cpp:14-21: complex float in/out) */
+int umpa_fp_init(umpa_fp_op *op, const umpa_fp_desc *desc, const umpa_fp_settings *settings);
+/* AFP::Execute (fp_cpu.cpp:25-46 / fp_cuda.cpp:144-198): in/out complex float device
+ * memory, filter float [out.y][out.x] (NULL unless applyFilter) */
+int umpa_fp_execute(umpa_fp_op *op, const void *in, uint64_t inCount, void *out,
+                    uint64_t outCount, const float *filter);
+int umpa_fp_synchronize(umpa_fp_op *op);
+int umpa_fp_cleanup(umpa_fp_op *op);
+void umpa_fp_destroy(umpa_fp_op *op);
+
+/* ---- misc --------------------------------------------------------------------------------- */
+const char *umpa_status_string(int status);
+/* library version, for the loader's sanity check */
+int umpa_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

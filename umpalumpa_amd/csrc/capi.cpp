@@ -1,0 +1,449 @@
+// C ABI of umpalumpa_amd: operation handles, validation with the reference's Init/Execute
+// semantics, and dispatch to the gfx950 kernels.  Nothing here throws across the ABI.
+#include <hip/hip_runtime_api.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fr_host.hpp"
+#include "launch.hpp"
+#include "umpalumpa_amd.h"
+
+namespace {
+
+// Makes `device` current for the lifetime of the guard.
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = false;
+  explicit DeviceGuard(int device)
+  {
+    if (hipGetDevice(&prev) != hipSuccess) return;
+    ok = (prev == device) || hipSetDevice(device) == hipSuccess;
+  }
+  ~DeviceGuard()
+  {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+int from_hip(hipError_t e)
+{
+  if (e == hipSuccess) return UMPA_OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return UMPA_ERR_OUT_OF_MEMORY;
+  return UMPA_ERR_DEVICE;
+}
+
+int query_cus(int device)
+{
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus;
+}
+
+// Size::IsValid (data/size.hpp:43-57, 86): n >= 1 and a valid 1/2/3-D shape
+bool size_valid(const umpa_size &s)
+{
+  if (s.n < 1) return false;
+  const bool d3 = s.z >= 2 && s.y >= 2 && s.x >= 2;
+  const bool d2 = s.z == 1 && s.y >= 2 && s.x >= 2;
+  const bool d1 = s.z == 1 && s.y == 1 && s.x >= 1;
+  return d3 || d2 || d1;
+}
+
+bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+}  // namespace
+
+struct umpa_fr_op {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int numCUs = 256;
+  bool initialized = false;
+  umpa_fr_desc desc{};
+  umpa_fr_settings settings{};
+  umpa_fr_constants constants{};
+  int strategy = UMPA_FR_EXACT;
+  bool countUpdates = false;
+  unsigned long long lastUpdates = 0;
+  int *dBricks = nullptr;
+  int nBricks = 0;
+  int *dCounter = nullptr;
+  unsigned long long *dUpdates = nullptr;
+  int nb = 0;
+
+  void release()
+  {
+    if (dBricks) (void)hipFree(dBricks);
+    if (dCounter) (void)hipFree(dCounter);
+    if (dUpdates) (void)hipFree(dUpdates);
+    dBricks = nullptr;
+    dCounter = nullptr;
+    dUpdates = nullptr;
+    nBricks = 0;
+    initialized = false;
+  }
+};
+
+struct umpa_fp_op {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int numCUs = 256;
+  bool initialized = false;
+  umpa_fp_desc desc{};
+  umpa_fp_settings settings{};
+};
+
+extern "C" {
+
+int umpa_version(void) { return 100; }
+
+const char *umpa_status_string(int status)
+{
+  switch (status) {
+  case UMPA_OK: return "ok";
+  case UMPA_ERR_INVALID_ARG: return "invalid argument";
+  case UMPA_ERR_NOT_INITIALIZED: return "operation not initialized";
+  case UMPA_ERR_NOT_EQUIVALENT: return "payload not equivalent to the initialized one";
+  case UMPA_ERR_REJECTED: return "execution rejected (the reference would return false)";
+  case UMPA_ERR_UNSUPPORTED: return "unsupported";
+  case UMPA_ERR_DEVICE: return "HIP device error";
+  case UMPA_ERR_OUT_OF_MEMORY: return "out of device memory";
+  default: return "unknown status";
+  }
+}
+
+// ---- FR ---------------------------------------------------------------------------------
+int umpa_fr_create(umpa_fr_op **op, int device, void *stream)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  *op = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return UMPA_ERR_DEVICE;
+  auto *h = new (std::nothrow) umpa_fr_op();
+  if (!h) return UMPA_ERR_OUT_OF_MEMORY;
+  h->device = device;
+  h->stream = static_cast<hipStream_t>(stream);
+  h->numCUs = query_cus(device);
+  *op = h;
+  return UMPA_OK;
+}
+
+int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_settings *s)
+{
+  if (!op || !desc || !s) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  // Init starts from a clean state (basic_operation.hpp:30-31)
+  op->release();
+  // IsValid (afr.hpp:47-54) and Strategy1::Init (fr_cpu.cpp:14-19): odd volume x only
+  if (!size_valid(desc->volume) || !size_valid(desc->fft)) return UMPA_ERR_INVALID_ARG;
+  if (desc->volume.x % 2 == 0) return UMPA_ERR_INVALID_ARG;
+  // the volume is indexed with one extent for y and z (fr_cpu_kernels.hpp:114-115)
+  if (desc->volume.y != desc->volume.z || desc->volume.n != 1) return UMPA_ERR_INVALID_ARG;
+  if (s->interpolation < 0 || s->interpolation > 1 || s->type < 0 || s->type > 1 ||
+      s->order < 0 || s->order > 4 || !(s->blobRadius > 0.f) || !std::isfinite(s->alpha))
+    return UMPA_ERR_INVALID_ARG;
+  if (desc->volume.x > (uint64_t)INT_MAX || desc->volume.y > (uint64_t)INT_MAX ||
+      desc->fft.x > (uint64_t)INT_MAX || desc->fft.y > (uint64_t)INT_MAX ||
+      desc->fft.n > (uint64_t)INT_MAX || desc->blobTable > (uint64_t)INT_MAX)
+    return UMPA_ERR_UNSUPPORTED;
+  try {
+    op->desc = *desc;
+    op->settings = *s;
+    op->constants = umpa::host::create_constants(desc->volume.x, desc->volume.y,
+                                                 desc->blobTable, *s);
+    const int vx = (int)desc->volume.x, vy = (int)desc->volume.y, vz = (int)desc->volume.z;
+    auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, 4, 4, 8);
+    std::vector<int> flat(bricks.size() * 4);
+    for (size_t i = 0; i < bricks.size(); ++i) {
+      flat[4 * i + 0] = bricks[i].x;
+      flat[4 * i + 1] = bricks[i].y;
+      flat[4 * i + 2] = bricks[i].z;
+      flat[4 * i + 3] = bricks[i].w;
+    }
+    hipError_t e = hipMalloc(&op->dBricks, flat.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&op->dCounter, sizeof(int));
+    if (e == hipSuccess) e = hipMalloc(&op->dUpdates, sizeof(unsigned long long));
+    if (e == hipSuccess)
+      e = hipMemcpy(op->dBricks, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      op->release();
+      return from_hip(e);
+    }
+    op->nBricks = (int)bricks.size();
+    const double span = 2.0 * (double)s->blobRadius;
+    op->nb = span < 5.5 ? (int)std::floor(span) + 1 : 0;
+    if (op->nb < 4) op->nb = 4;
+    op->initialized = true;
+    return UMPA_OK;
+  } catch (const std::bad_alloc &) {
+    op->release();
+    return UMPA_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    op->release();
+    return UMPA_ERR_DEVICE;
+  }
+}
+
+static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales, bool half,
+                           uint64_t fftCount, void *volume, float *weights,
+                           const umpa_traverse_space *spaces, uint64_t nSpaces,
+                           const float *table)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  if (!op->initialized) return UMPA_ERR_NOT_INITIALIZED;
+  // BasicOperation::Execute equivalence (basic_operation.hpp:62-63): N may shrink
+  if (fftCount > op->desc.fft.n) return UMPA_ERR_NOT_EQUIVALENT;
+  // FRCUDA Strategy1::Execute rejects empty payloads (fr_cuda.cpp:165-167)
+  if (!fft || !volume || !weights || fftCount == 0) return UMPA_ERR_REJECTED;
+  // FRCPU rejects kLookup without a table (fr_cpu.cpp:27-29)
+  if (op->settings.interpolation == UMPA_FR_LOOKUP && (!table || op->desc.blobTable == 0))
+    return UMPA_ERR_REJECTED;
+  if (nSpaces > (uint64_t)INT_MAX) return UMPA_ERR_UNSUPPORTED;
+  if (nSpaces > 0 && !spaces) return UMPA_ERR_INVALID_ARG;
+  if (!aligned(volume, 8) || !aligned(weights, 4) || !aligned(fft, half ? 4 : 8) ||
+      !aligned(spaces, 4) || (table && !aligned(table, 4)))
+    return UMPA_ERR_INVALID_ARG;
+  if (half && !scales) return UMPA_ERR_INVALID_ARG;
+  op->lastUpdates = 0;
+  if (nSpaces == 0) return UMPA_OK;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+
+  umpa::FrArgs a{};
+  a.volume = volume;
+  a.weights = weights;
+  a.fft = fft;
+  a.scales = scales;
+  a.xSize = (int)op->desc.fft.x;
+  a.ySize = (int)op->desc.fft.y;
+  a.sliceElems = (int64_t)op->desc.fft.x * (int64_t)op->desc.fft.y;
+  a.nProj = (int)fftCount;
+  a.nSpaces = (int)nSpaces;
+  a.spaces = spaces;
+  a.table = table;
+  a.nTable = (int)op->desc.blobTable;
+  a.c = op->constants;
+  a.vx = (int)op->desc.volume.x;
+  a.vy = (int)op->desc.volume.y;
+  a.vz = (int)op->desc.volume.z;
+  a.bricks = op->dBricks;
+  a.nBricks = op->nBricks;
+  a.counter = op->dCounter;
+  a.updates = op->dUpdates;
+  const bool nearest = op->settings.type == UMPA_FR_FAST;
+  a.cullHalfWidth = nearest ? 1.f : op->settings.blobRadius;
+  a.cullHalfSpace = nearest ? 0.f : op->settings.blobRadius;
+
+  umpa::FrVariant v{};
+  if (nearest) {
+    v.mode = umpa::kModeNearest;
+  } else if (op->settings.interpolation == UMPA_FR_LOOKUP) {
+    v.mode = umpa::kModeLookup;
+  } else {
+    // FRCPU picks the rational I0 approximation for alpha <= 15 (fr_cpu.cpp:31)
+    v.mode = op->settings.alpha <= 15.f ? umpa::kModeFastKaiser : umpa::kModePreciseKaiser;
+  }
+  v.order = op->settings.order;
+  v.nb = op->nb;
+  v.half = half;
+  v.fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
+  v.count = op->countUpdates;
+  if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
+  hipError_t e = hipSuccess;
+  if (v.count) e = hipMemsetAsync(op->dUpdates, 0, sizeof(unsigned long long), op->stream);
+  if (e == hipSuccess) e = umpa::launch_fr_insert(a, v, op->numCUs, op->stream);
+  if (e == hipSuccess && v.count) {
+    e = hipMemcpyAsync(&op->lastUpdates, op->dUpdates, sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, op->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(op->stream);
+  }
+  return from_hip(e);
+}
+
+int umpa_fr_execute(umpa_fr_op *op, const void *fft, uint64_t fftCount, void *volume,
+                    float *weights, const umpa_traverse_space *spaces, uint64_t nSpaces,
+                    const float *table)
+{
+  return fr_execute_impl(op, fft, nullptr, false, fftCount, volume, weights, spaces, nSpaces,
+                         table);
+}
+
+int umpa_fr_execute_f16(umpa_fr_op *op, const void *fft, const float *scales,
+                        uint64_t fftCount, void *volume, float *weights,
+                        const umpa_traverse_space *spaces, uint64_t nSpaces,
+                        const float *table)
+{
+  return fr_execute_impl(op, fft, scales, true, fftCount, volume, weights, spaces, nSpaces,
+                         table);
+}
+
+int umpa_fr_set_strategy(umpa_fr_op *op, int strategy)
+{
+  if (!op || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH))
+    return UMPA_ERR_INVALID_ARG;
+  op->strategy = strategy;
+  return UMPA_OK;
+}
+
+int umpa_fr_set_count_updates(umpa_fr_op *op, int enable)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  op->countUpdates = enable != 0;
+  return UMPA_OK;
+}
+
+int umpa_fr_last_updates(umpa_fr_op *op, uint64_t *updates)
+{
+  if (!op || !updates) return UMPA_ERR_INVALID_ARG;
+  *updates = op->lastUpdates;
+  return UMPA_OK;
+}
+
+int umpa_fr_synchronize(umpa_fr_op *op)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  return from_hip(hipStreamSynchronize(op->stream));
+}
+
+int umpa_fr_cleanup(umpa_fr_op *op)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (op->initialized) (void)hipStreamSynchronize(op->stream);
+  op->release();
+  return UMPA_OK;
+}
+
+void umpa_fr_destroy(umpa_fr_op *op)
+{
+  if (!op) return;
+  umpa_fr_cleanup(op);
+  delete op;
+}
+
+int umpa_fr_create_constants(const umpa_fr_desc *desc, const umpa_fr_settings *s,
+                             umpa_fr_constants *out)
+{
+  if (!desc || !s || !out) return UMPA_ERR_INVALID_ARG;
+  *out = umpa::host::create_constants(desc->volume.x, desc->volume.y, desc->blobTable, *s);
+  return UMPA_OK;
+}
+
+int umpa_fr_fill_blob_table(float *table, uint64_t n, const umpa_fr_desc *desc,
+                            const umpa_fr_settings *s)
+{
+  if (!table || !desc || !s || n == 0) return UMPA_ERR_INVALID_ARG;
+  umpa::host::fill_blob_table(table, n, desc->volume.x, desc->volume.y, *s);
+  return UMPA_OK;
+}
+
+int umpa_fr_compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY,
+                                   const float transform[9], umpa_traverse_space *space,
+                                   uint32_t maxVolumeIndexX, uint32_t maxVolumeIndexYZ,
+                                   int useFast, float blobRadius, float weight)
+{
+  if (!transform || !space) return UMPA_ERR_INVALID_ARG;
+  umpa::host::compute_traverse_space(imgSizeX, imgSizeY, transform, *space, maxVolumeIndexX,
+                                     maxVolumeIndexYZ, useFast != 0, blobRadius, weight);
+  return UMPA_OK;
+}
+
+// ---- FP ---------------------------------------------------------------------------------
+int umpa_fp_create(umpa_fp_op **op, int device, void *stream)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  *op = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return UMPA_ERR_DEVICE;
+  auto *h = new (std::nothrow) umpa_fp_op();
+  if (!h) return UMPA_ERR_OUT_OF_MEMORY;
+  h->device = device;
+  h->stream = static_cast<hipStream_t>(stream);
+  h->numCUs = query_cus(device);
+  *op = h;
+  return UMPA_OK;
+}
+
+int umpa_fp_init(umpa_fp_op *op, const umpa_fp_desc *desc, const umpa_fp_settings *s)
+{
+  if (!op || !desc || !s) return UMPA_ERR_INVALID_ARG;
+  op->initialized = false;
+  // AFP::IsValid (afp.hpp:45-52) + Settings::SetMaxFreq's range (settings.hpp:38-43)
+  if (!size_valid(desc->in) || !size_valid(desc->out) || !size_valid(desc->inSpatial))
+    return UMPA_ERR_INVALID_ARG;
+  if (desc->out.n < desc->in.n) return UMPA_ERR_INVALID_ARG;
+  if (desc->out.x > desc->in.x || desc->out.y > desc->in.y) return UMPA_ERR_UNSUPPORTED;
+  if (s->hasMaxFreq && !(s->maxFreq >= 0.f && s->maxFreq <= 0.5f)) return UMPA_ERR_INVALID_ARG;
+  if (desc->inPaddedSpatialSingle == 0) return UMPA_ERR_INVALID_ARG;
+  op->desc = *desc;
+  op->settings = *s;
+  op->initialized = true;
+  return UMPA_OK;
+}
+
+int umpa_fp_execute(umpa_fp_op *op, const void *in, uint64_t inCount, void *out,
+                    uint64_t outCount, const float *filter)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  if (!op->initialized) return UMPA_ERR_NOT_INITIALIZED;
+  if (inCount > op->desc.in.n || outCount > op->desc.out.n) return UMPA_ERR_NOT_EQUIVALENT;
+  // FPCPU Strategy1::Execute rejects empty payloads (fp_cpu.cpp:27-29)
+  if (!in || !out || inCount == 0 || outCount == 0) return UMPA_ERR_REJECTED;
+  if (outCount < inCount) return UMPA_ERR_INVALID_ARG;
+  if (op->settings.applyFilter && !filter) return UMPA_ERR_INVALID_ARG;
+  if (!aligned(in, 8) || !aligned(out, 8) || (filter && !aligned(filter, 4)))
+    return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  umpa::FpArgs a{};
+  a.in = in;
+  a.out = out;
+  a.filter = filter;
+  a.inX = op->desc.in.x;
+  a.inY = op->desc.in.y;
+  a.inSingle = op->desc.in.x * op->desc.in.y * op->desc.in.z;
+  a.spatialX = op->desc.inSpatial.x;
+  a.spatialY = op->desc.inSpatial.y;
+  a.outX = op->desc.out.x;
+  a.outY = op->desc.out.y;
+  a.outSingle = op->desc.out.x * op->desc.out.y * op->desc.out.z;
+  a.n = inCount;
+  a.normFactor = 1.f / static_cast<float>(op->desc.inPaddedSpatialSingle);
+  a.maxFreqSquare = op->settings.hasMaxFreq ? op->settings.maxFreq : 0.f;
+  a.applyFilter = op->settings.applyFilter != 0;
+  a.normalize = op->settings.normalize != 0;
+  a.center = op->settings.center != 0;
+  a.cropFreq = op->settings.hasMaxFreq != 0;
+  a.shift = op->settings.shift != 0;
+  return from_hip(umpa::launch_fp_scale(a, op->numCUs, op->stream));
+}
+
+int umpa_fp_synchronize(umpa_fp_op *op)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  return from_hip(hipStreamSynchronize(op->stream));
+}
+
+int umpa_fp_cleanup(umpa_fp_op *op)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  op->initialized = false;
+  return UMPA_OK;
+}
+
+void umpa_fp_destroy(umpa_fp_op *op) { delete op; }
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// FP crop / filter / normalize / center / shift kernel for gfx950.
+//
+// The reference CUDA kernel gives each thread one output pixel and loops over all images
+// (fp_cuda_kernels.cu:9-61).  This is a pure HBM stream, so here every lane owns one complex
+// output element of the flattened [n][y][x] output, consecutive lanes take consecutive x, and
+// a grid-stride loop covers any batch size: loads of an input row and stores of an output row
+// are both contiguous.  Arithmetic and filter indexing follow the reference CPU path
+// (ScaleFFT2DCPU, fp_cpu_kernels.hpp:8-41), compiled without contraction, so the output bits
+// equal FPCPU's.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "launch.hpp"
+
+namespace umpa {
+
+
+// Idx2Freq (fp_common_kernels.hpp:13-20): digital frequency of an FFT index
+__device__ __forceinline__ float idx2freq(uint64_t idx, uint64_t size)
+{
+  if (size <= 1) return 0.f;
+  float tmp = (idx <= (size / 2)) ? (float)idx : ((float)idx - (float)size);
+  return tmp / (float)size;
+}
+
+__global__ __launch_bounds__(256) void fp_scale_kernel(FpArgs a)
+{
+  const uint64_t total = a.n * a.outSingle;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    const uint64_t n = e / a.outSingle;
+    const uint64_t rem = e - n * a.outSingle;
+    const uint64_t y = rem / a.outX;
+    const uint64_t x = rem - y * a.outX;
+    // low frequencies sit in the corners: keep the top outY/2+1 rows and the bottom ones
+    const uint64_t origY = (y <= a.outY / 2) ? y : (a.inY - (a.outY - y));
+    const uint64_t tmp = y + a.outY / 2;
+    const uint64_t shiftedY = tmp >= a.outY ? tmp - a.outY : tmp;
+    const uint64_t o = n * a.outSingle + (a.shift ? shiftedY : y) * a.outX + x;
+    float re, im;
+    bool zero = false;
+    if (a.cropFreq) {
+      // |f|^2 as std::norm(std::complex<float>) computes it: the square of hypotf
+      const float fx = idx2freq(x, a.spatialX), fy = idx2freq(origY, a.spatialY);
+      const float mag = (float)sqrt((double)fx * (double)fx + (double)fy * (double)fy);
+      zero = (mag * mag) > a.maxFreqSquare;
+    }
+    if (zero) {
+      re = 0.f;
+      im = 0.f;
+    } else {
+      const float2 v = reinterpret_cast<const float2 *>(a.in)[n * a.inSingle + origY * a.inX + x];
+      re = v.x;
+      im = v.y;
+    }
+    if (a.applyFilter) {
+      const float f = a.filter[y * a.outX + x];
+      re *= f;
+      im *= f;
+    }
+    if (a.normalize) {
+      re *= a.normFactor;
+      im *= a.normFactor;
+    }
+    if (a.center) {
+      const float cc = (float)(1 - 2 * (((int)x + (int)y) & 1));
+      re *= cc;
+      im *= cc;
+    }
+    reinterpret_cast<float2 *>(a.out)[o] = make_float2(re, im);
+  }
+}
+
+hipError_t launch_fp_scale(const FpArgs &a, int numCUs, hipStream_t st)
+{
+  const uint64_t total = a.n * a.outSingle;
+  if (total == 0) return hipSuccess;
+  uint64_t blocks = (total + 255) / 256;
+  const uint64_t cap = (uint64_t)numCUs * 32;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(fp_scale_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace umpa

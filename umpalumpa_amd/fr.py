@@ -1,0 +1,199 @@
+"""Python mirror of the reference's AFR / AFP operations over the C ABI.
+
+``FourierReconstruction`` follows AFR (fourier_reconstruction/afr.hpp:13-55) and
+``FourierProcessing`` follows AFP (fourier_processing/afp.hpp:13-53): ``init`` and ``execute``
+return ``True``/``False`` like the reference's ``[[nodiscard]] bool Init/Execute``
+(basic_operation.hpp:28-66), ``synchronize`` waits for the enqueued work and ``cleanup``
+drops the Init state.  Device buffers are torch tensors on the operation's GPU; the work is
+enqueued on torch's current stream of that device unless a stream is given.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (FR_DYNAMIC, FR_EXACT, FR_FAST, FR_FAST_MATH, FR_LOOKUP, FR_PRECISE,
+                   TRAVERSE_SPACE_DTYPE, FPDesc, FPSettings, FRConstants, FRDesc, FRSettings,
+                   Size, check)
+
+__all__ = [
+    "FR_DYNAMIC", "FR_LOOKUP", "FR_FAST", "FR_PRECISE", "FR_EXACT", "FR_FAST_MATH",
+    "TRAVERSE_SPACE_DTYPE", "FRSettings", "FRDesc", "FPSettings", "Size",
+    "FourierReconstruction", "FourierProcessing", "create_constants", "fill_blob_table",
+    "compute_traverse_space", "traverse_spaces",
+]
+
+
+def _stream_handle(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def fr_desc(slice_x: int, slice_y: int, n_slices: int, volume: int, n_spaces: int = 0,
+            n_table: int = 0) -> FRDesc:
+    """Descriptor of AFR's payloads: slices Hx x Hy (n of them), volume^3, spaces, table."""
+    return FRDesc(Size(slice_x, slice_y, 1, n_slices), Size(volume, volume, volume, 1),
+                  n_spaces, n_table)
+
+
+# ---- host statics (no GPU) ----------------------------------------------------------------
+def create_constants(desc: FRDesc, s: FRSettings) -> FRConstants:
+    """AFR::CreateConstants (afr.cpp:6-19)."""
+    out = FRConstants()
+    check(_lib.load().umpa_fr_create_constants(C.byref(desc), C.byref(s), C.byref(out)),
+          "create_constants")
+    return out
+
+
+def fill_blob_table(n: int, desc: FRDesc, s: FRSettings) -> np.ndarray:
+    """AFR::FillBlobTable (afr.cpp:22-34) into a new host array of n floats."""
+    t = np.zeros(n, dtype=np.float32)
+    check(_lib.load().umpa_fr_fill_blob_table(t.ctypes.data_as(C.c_void_p), n, C.byref(desc),
+                                              C.byref(s)), "fill_blob_table")
+    return t
+
+
+def compute_traverse_space(img_x: int, img_y: int, transform, max_x: int, max_yz: int,
+                           use_fast: bool, blob_radius: float, weight: float = 1.0):
+    """ComputeTraverseSpace (traverse_space_generator.hpp:94-145) for one 3x3 transform."""
+    t = np.ascontiguousarray(transform, dtype=np.float32).reshape(9)
+    out = np.zeros(1, dtype=TRAVERSE_SPACE_DTYPE)
+    check(_lib.load().umpa_fr_compute_traverse_space(
+        img_x, img_y, t.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), max_x,
+        max_yz, int(use_fast), blob_radius, weight), "compute_traverse_space")
+    return out[0]
+
+
+def traverse_spaces(rotations, padded_box: int, s: FRSettings, symmetries=None,
+                    weight: float = 1.0) -> np.ndarray:
+    """Traverse spaces of a chunk the way the reference caller builds them
+    (examples/fourier_reconstruction/fr.cpp:278-312): for every slice i and every symmetry
+    operator S, transform = R_i @ S (FP64 on the host, cast to float), projectionIndex = i.
+    The slice extent along x is padded_box/2 (afr_common.hpp:57-59)."""
+    rot = np.asarray(rotations, dtype=np.float64).reshape(-1, 3, 3)
+    sym = np.eye(3)[None] if symmetries is None else np.asarray(symmetries, np.float64)
+    out = np.zeros(len(rot) * len(sym), dtype=TRAVERSE_SPACE_DTYPE)
+    k = 0
+    for i, r in enumerate(rot):
+        for s_op in sym:
+            rec = compute_traverse_space(padded_box // 2, padded_box, (r @ s_op), padded_box,
+                                         padded_box, s.type == FR_FAST, s.blobRadius, weight)
+            rec["projectionIndex"] = i
+            out[k] = rec
+            k += 1
+    return out
+
+
+# ---- AFR ----------------------------------------------------------------------------------
+class FourierReconstruction:
+    """AFR on one GPU: hand-written gfx950 voxel-owned gather kernels."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self.device = device
+        self._stream = stream if stream is not None else torch.cuda.current_stream(device)
+        self._h = C.c_void_p()
+        check(_lib.load().umpa_fr_create(C.byref(self._h), device, _stream_handle(self._stream)),
+              "umpa_fr_create")
+        self.desc = None
+        self.settings = None
+        self.last_status = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().umpa_fr_destroy(h)
+            self._h = None
+
+    def init(self, desc: FRDesc, settings: FRSettings) -> bool:
+        self.last_status = _lib.load().umpa_fr_init(self._h, C.byref(desc), C.byref(settings))
+        ok = self.last_status == 0
+        self.desc, self.settings = (desc, settings) if ok else (None, None)
+        return ok
+
+    def set_strategy(self, strategy: int) -> None:
+        check(_lib.load().umpa_fr_set_strategy(self._h, strategy), "set_strategy")
+
+    def set_count_updates(self, enable: bool) -> None:
+        check(_lib.load().umpa_fr_set_count_updates(self._h, int(enable)), "set_count_updates")
+
+    def last_updates(self) -> int:
+        v = C.c_uint64(0)
+        check(_lib.load().umpa_fr_last_updates(self._h, C.byref(v)), "last_updates")
+        return int(v.value)
+
+    def execute(self, fft, volume, weights, spaces, table=None, n_spaces=None,
+                scales=None) -> bool:
+        """fft: complex64 [n, Hy, Hx] (or float16 [n, Hy, Hx, 2] with `scales`), volume:
+        complex64 [Z, Y, X], weights: float32 [Z, Y, X], spaces: uint8 device tensor of
+        112-byte records, table: float32 device tensor."""
+        n_sp = spaces.numel() // 112 if n_spaces is None else n_spaces
+        lib = _lib.load()
+        if scales is None:
+            self.last_status = lib.umpa_fr_execute(
+                self._h, _ptr(fft), fft.shape[0], _ptr(volume), _ptr(weights), _ptr(spaces),
+                n_sp, _ptr(table))
+        else:
+            self.last_status = lib.umpa_fr_execute_f16(
+                self._h, _ptr(fft), _ptr(scales), fft.shape[0], _ptr(volume), _ptr(weights),
+                _ptr(spaces), n_sp, _ptr(table))
+        return self.last_status == 0
+
+    def synchronize(self) -> None:
+        check(_lib.load().umpa_fr_synchronize(self._h), "synchronize")
+
+    def cleanup(self) -> None:
+        check(_lib.load().umpa_fr_cleanup(self._h), "cleanup")
+        self.desc = self.settings = None
+
+
+# ---- AFP ----------------------------------------------------------------------------------
+class FourierProcessing:
+    """AFP on one GPU (out-of-place crop / filter / normalize / center / shift)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self.device = device
+        self._stream = stream if stream is not None else torch.cuda.current_stream(device)
+        self._h = C.c_void_p()
+        check(_lib.load().umpa_fp_create(C.byref(self._h), device, _stream_handle(self._stream)),
+              "umpa_fp_create")
+        self.last_status = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().umpa_fp_destroy(h)
+            self._h = None
+
+    def init(self, desc: FPDesc, settings: FPSettings) -> bool:
+        self.last_status = _lib.load().umpa_fp_init(self._h, C.byref(desc), C.byref(settings))
+        return self.last_status == 0
+
+    def execute(self, inp, in_count, out, out_count, filt=None) -> bool:
+        self.last_status = _lib.load().umpa_fp_execute(self._h, _ptr(inp), in_count, _ptr(out),
+                                                       out_count, _ptr(filt))
+        return self.last_status == 0
+
+    def synchronize(self) -> None:
+        check(_lib.load().umpa_fp_synchronize(self._h), "synchronize")
+
+    def cleanup(self) -> None:
+        check(_lib.load().umpa_fp_cleanup(self._h), "cleanup")
+
+
+def fp_desc(in_spatial, out_spatial, n: int, padded_spatial_single: int | None = None) -> FPDesc:
+    """Descriptor for a crop of spatial (x, y) images to (x', y'): frequency sizes x/2+1."""
+    ix, iy = in_spatial
+    ox, oy = out_spatial
+    single = ix * iy if padded_spatial_single is None else padded_spatial_single
+    return FPDesc(Size(ix // 2 + 1, iy, 1, n), Size(ix, iy, 1, n), Size(ox // 2 + 1, oy, 1, n),
+                  single)
