@@ -178,9 +178,10 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
       return from_hip(e);
     }
     op->nBricks = (int)bricks.size();
-    const double span = 2.0 * (double)s->blobRadius;
-    op->nb = span < 5.5 ? (int)std::floor(span) + 1 : 0;
-    if (op->nb < 4) op->nb = 4;
+    // widest blob footprint in pixels: floor(2r) + 1 (plus a margin for rounding at the edge);
+    // wider footprints take the kernel's generic path
+    const int nb = (int)std::floor(2.0 * (double)s->blobRadius + 1e-3) + 1;
+    op->nb = nb < 3 ? 3 : (nb > 6 ? 6 : nb);
     op->initialized = true;
     return UMPA_OK;
   } catch (const std::bad_alloc &) {
