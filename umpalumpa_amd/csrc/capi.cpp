@@ -71,8 +71,10 @@ struct umpa_fr_op {
   int strategy = UMPA_FR_EXACT;
   bool countUpdates = false;
   unsigned long long lastUpdates = 0;
-  int *dBricks = nullptr;
+  int *dBricks = nullptr;      // exact schedule: every brick owned by one wavefront
   int nBricks = 0;
+  int *dBricksSplit = nullptr; // fast-math schedule: central bricks split over wavefronts
+  int nBricksSplit = 0;
   int *dCounter = nullptr;
   unsigned long long *dUpdates = nullptr;
   int nb = 0;
@@ -80,9 +82,12 @@ struct umpa_fr_op {
   void release()
   {
     if (dBricks) (void)hipFree(dBricks);
+    if (dBricksSplit) (void)hipFree(dBricksSplit);
     if (dCounter) (void)hipFree(dCounter);
     if (dUpdates) (void)hipFree(dUpdates);
     dBricks = nullptr;
+    dBricksSplit = nullptr;
+    nBricksSplit = 0;
     dCounter = nullptr;
     dUpdates = nullptr;
     nBricks = 0;
@@ -160,23 +165,30 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
     op->constants = umpa::host::create_constants(desc->volume.x, desc->volume.y,
                                                  desc->blobTable, *s);
     const int vx = (int)desc->volume.x, vy = (int)desc->volume.y, vz = (int)desc->volume.z;
-    auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, 4, 4, 8);
-    std::vector<int> flat(bricks.size() * 4);
-    for (size_t i = 0; i < bricks.size(); ++i) {
-      flat[4 * i + 0] = bricks[i].x;
-      flat[4 * i + 1] = bricks[i].y;
-      flat[4 * i + 2] = bricks[i].z;
-      flat[4 * i + 3] = bricks[i].w;
-    }
-    hipError_t e = hipMalloc(&op->dBricks, flat.size() * sizeof(int));
+    auto upload = [](const std::vector<umpa::host::BrickOrigin> &bricks, int **dst) {
+      std::vector<int> flat(bricks.size() * 4);
+      for (size_t i = 0; i < bricks.size(); ++i) {
+        flat[4 * i + 0] = bricks[i].x;
+        flat[4 * i + 1] = bricks[i].y;
+        flat[4 * i + 2] = bricks[i].z;
+        flat[4 * i + 3] = bricks[i].w;
+      }
+      hipError_t e = hipMalloc(dst, flat.size() * sizeof(int));
+      if (e == hipSuccess)
+        e = hipMemcpy(*dst, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
+      return e;
+    };
+    auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, false);
+    auto split = umpa::host::make_brick_schedule(vx, vy, vz, true);
+    hipError_t e = upload(bricks, &op->dBricks);
+    if (e == hipSuccess) e = upload(split, &op->dBricksSplit);
     if (e == hipSuccess) e = hipMalloc(&op->dCounter, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&op->dUpdates, sizeof(unsigned long long));
-    if (e == hipSuccess)
-      e = hipMemcpy(op->dBricks, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
       op->release();
       return from_hip(e);
     }
+    op->nBricksSplit = (int)split.size();
     op->nBricks = (int)bricks.size();
     // widest blob footprint in pixels: floor(2r) + 1 (plus a margin for rounding at the edge);
     // wider footprints take the kernel's generic path
@@ -235,12 +247,14 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
   a.vx = (int)op->desc.volume.x;
   a.vy = (int)op->desc.volume.y;
   a.vz = (int)op->desc.volume.z;
-  a.bricks = op->dBricks;
-  a.nBricks = op->nBricks;
+  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
+  const bool fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
+  a.bricks = fastMath ? op->dBricksSplit : op->dBricks;
+  a.nBricks = fastMath ? op->nBricksSplit : op->nBricks;
   a.counter = op->dCounter;
   a.updates = op->dUpdates;
   const bool nearest = op->settings.type == UMPA_FR_FAST;
-  a.cullHalfWidth = nearest ? 1.f : op->settings.blobRadius;
+  a.cullHalfWidth = nearest ? 1.6f : op->settings.blobRadius;
   a.cullHalfSpace = nearest ? 0.f : op->settings.blobRadius;
 
   umpa::FrVariant v{};

@@ -217,21 +217,61 @@ void compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY, const float tr
   space.weight = weight;
 }
 
-std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, int bx, int by, int bz)
+std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy)
 {
-  std::vector<BrickOrigin> bricks;
+  // Bricks are 8 x 8 x D columns, D in {8, 4, 2, 1}.  A voxel at distance R from the centre
+  // is crossed by a fraction ~ r/R of random slices, so the work of a brick grows towards the
+  // centre; D shrinks with R to even the work per brick out.
+  struct Cand {
+    int x, y, z, d;
+    float est;
+  };
+  std::vector<Cand> cand;
   const float cx = static_cast<float>((vx - 1) / 2), cy = static_cast<float>((vy - 1) / 2);
-  for (int z = 0; z < vz; z += bz)
-    for (int y = 0; y < vy; y += by)
-      for (int x = 0; x < vx; x += bx) {
-        const float dx = x + 0.5f * (bx - 1) - cx, dy = y + 0.5f * (by - 1) - cy,
-                    dz = z + 0.5f * (bz - 1) - cy;
-        bricks.push_back({ x, y, z, static_cast<int>(std::sqrt(dx * dx + dy * dy + dz * dz)) });
+  auto radius = [&](int x, int y, int z, int d) {
+    const float dx = x + 3.5f - cx, dy = y + 3.5f - cy, dz = z + 0.5f * (d - 1) - cy;
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  auto depth_for = [](float R) { return R > 48.f ? 8 : R > 24.f ? 4 : R > 12.f ? 2 : 1; };
+  double total = 0;
+  for (int y = 0; y < vy; y += 8)
+    for (int x = 0; x < vx; x += 8)
+      for (int z = 0; z < vz;) {
+        int d = 8;
+        while (d > 1 && depth_for(radius(x, y, z, d)) < d) d /= 2;
+        const float R = radius(x, y, z, d);
+        // expected cost per slice: P(active) times the cost of an active (brick, slice) pair.
+        // A random plane passes within w of a point at distance R with probability w/R; away
+        // from the centre the half-space test removes half of those.
+        const float w = 1.9f + 3.5f * 1.5f + 0.25f * (float)d;
+        const float pslab = w / std::max(R, w);
+        const float p = pslab * (0.5f + 0.5f * pslab);
+        const float passes = std::max(1.f, 0.3f * (float)d);  // ~30% of an active brick is in the slab
+        const float est = p * (68.f + 8.f * d + 300.f * passes) + 2.f;
+        cand.push_back({ x, y, z, d, est });
+        total += est;
+        z += d;
       }
-  // a brick at distance R from the centre is crossed by a fraction ~1/R of random slices:
-  // hand the central (heaviest) bricks out first
+  // a wavefront's share of the work on a full MI355X (16 wavefronts on each of 256 CUs); only
+  // bricks well above both that share and the typical brick are prioritised or split
+  const double fair = total / 4096.0;
+  const double mean = total / (double)std::max<size_t>(cand.size(), 1);
+  std::vector<BrickOrigin> bricks;
+  for (const Cand &c : cand) {
+    int flags = c.d;
+    if (c.est > fair / 4 && c.est > 4 * mean) flags |= 1 << 4;  // heavy: raised wave priority
+    int parts = 1;
+    if (splitHeavy && c.est > fair / 2 && c.est > 4 * mean)
+      parts = std::min(16, (int)std::ceil(c.est / (fair / 2)));
+    for (int j = 0; j < parts; ++j) {
+      BrickOrigin b{ c.x, c.y, c.z, flags | (parts << 8) | (j << 16), 0.f };
+      b.est = c.est / parts;
+      bricks.push_back(b);
+    }
+  }
+  // heaviest first, so the long central bricks start at once
   std::stable_sort(bricks.begin(), bricks.end(),
-    [](const BrickOrigin &a, const BrickOrigin &b) { return a.w < b.w; });
+    [](const BrickOrigin &a, const BrickOrigin &b) { return a.est > b.est; });
   return bricks;
 }
 

@@ -22,12 +22,16 @@ void compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY, const float tr
                             uint32_t maxVolumeIndexYZ, bool useFast, float blobRadius,
                             float weight);
 
-// Origin of a brick of the insertion kernel plus its scheduling key.
+// One work item of the insertion kernel: the origin of an 8 x 8 x D brick and
+// w = D | heavy << 4 | parts << 8 | part << 16 (a brick split into `parts` slice ranges is
+// accumulated by `parts` wavefronts, which merge their sums with atomics).
 struct BrickOrigin {
   int x, y, z, w;
+  float est;  // estimated cost, for ordering only
 };
 
-// Bricks of bx*by*bz voxels covering a vx*vy*vz volume, most central first.
-std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, int bx, int by, int bz);
+// Work items covering a vx*vy*vz volume, heaviest first.  `splitHeavy` splits the central
+// bricks' slice ranges (changes the summation order: fast-math strategies only).
+std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy);
 
 }  // namespace umpa::host

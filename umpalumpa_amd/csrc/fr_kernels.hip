@@ -1,33 +1,34 @@
 // FR insertion kernels for gfx950: voxel-owned gather.
 //
 // The reference scatters each rotated slice into the volume with three float atomics per voxel
-// (fr_cuda_kernels.cu:266-365).  Here a wavefront owns a brick of 4 x 4 x 8 voxels (two per
-// lane) for the whole launch and keeps their running (complex, weight) sums in registers.  It
-// culls the traverse spaces against the brick 64 at a time with one ballot and walks the
-// surviving spaces in index order, so every voxel receives its per-space contributions in the
-// order of the reference CPU loop over spaces (fr_cpu_kernels.hpp:254): no atomics, and no
+// (fr_cuda_kernels.cu:266-365).  Here a wavefront owns a brick of 8 x 8 x 8 voxels (a z column
+// of 8 per lane) for the whole launch and keeps their running (complex, weight) sums in its LDS
+// slot.  It culls the traverse spaces against the brick 64 at a time with one ballot and walks
+// the surviving spaces in index order, so every voxel receives its per-space contributions in
+// the order of the reference CPU loop over spaces (fr_cpu_kernels.hpp:254): no atomics, and no
 // dependence on how bricks are scheduled.
 //
 // Per (brick, space) pair the wave
-//   1. evaluates the reference's per-voxel predicate for its 128 voxels,
-//   2. compacts the voxels that pass (about a third) into a per-wave LDS queue, so the gather
-//      runs with full lanes,
-//   3. gathers the blob footprint of each queued voxel with a branch-free 4 x 4 (NB x NB)
-//      candidate loop: a candidate outside the blob reads the zero entry appended to the LDS
-//      blob table and an out-of-range buffer offset that the hardware returns as zero, which
-//      adds exactly +0 to the sums, so the result is bit-identical to the reference's
-//      skip-the-pixel loop while all table and pixel loads of an item issue back to back,
-//   4. hands each contribution back to the owning lane, which adds it to its register sum.
-// Workgroups are persistent and take bricks from a counter, most central (heaviest) first; the
-// blob table is staged once per workgroup in LDS.
+//   1. tests the reference's slab condition (q.z^2 <= r^2) for its 512 voxels, 4 VALU each,
+//      and compacts the voxels that pass (about a quarter) into its LDS queue;
+//   2. runs the rest of the reference predicate and the gather over the queue with full lanes:
+//      a branch-free NB x NB candidate loop in which a candidate the reference skips reads
+//      the zero entry appended to the LDS blob table and an out-of-range buffer offset that the
+//      hardware returns as zero, adding exactly +0, so the sums are bit-identical to the
+//      reference's skip-the-pixel loop while all table and pixel loads of an item issue back
+//      to back; each voxel appears once per space, so its lane adds the contribution to the
+//      voxel's LDS sum directly.
+// Workgroups are persistent (one per CU: 16 wavefronts sharing the blob table in LDS) and take
+// bricks from a counter, most central (heaviest) first.
 #include "fr_device.hpp"
 
 namespace umpa {
 
-constexpr int kMaxLdsTable = 16384;  // floats; larger tables are read from global memory
-constexpr int kWaves = 8;            // wavefronts per workgroup (they share the LDS table)
+constexpr int kMaxLdsTable = 12000;  // floats next to the per-wave state; larger tables stay global
+constexpr int kWaves = 16;           // wavefronts per workgroup (they share the LDS table)
 constexpr int kThreads = 64 * kWaves;
-constexpr int kQueue = 128;          // voxels per brick = queue slots per wave
+constexpr int kEdge = 8;             // brick edge: a wave owns 8 x 8 x 8 voxels, a z column per lane
+constexpr int kVox = kEdge * kEdge * kEdge;
 constexpr uint32_t kOutOfRange = 0x7ffff000u;  // buffer offset past every slice: loads read 0
 constexpr uint32_t kBufferConfig = 0x00020000u; // raw buffer descriptor word 3 on gfx9
 
@@ -57,13 +58,14 @@ __device__ uint32_t raw_buffer_load_u32(__amdgpu_buffer_rsrc_t rsrc, uint32_t vo
 // Slice pixel through a bounds-checked buffer resource: offsets at or past the slice end
 // return zero instead of faulting.
 template <bool kHalf>
-__device__ __forceinline__ float2 load_pixel(__amdgpu_buffer_rsrc_t rsrc, uint32_t byteOffset)
+__device__ __forceinline__ float2 load_pixel(__amdgpu_buffer_rsrc_t rsrc, uint32_t byteOffset,
+                                             uint32_t scalarOffset = 0)
 {
   if constexpr (kHalf) {
-    const uint32_t v = raw_buffer_load_u32(rsrc, byteOffset, 0, 0);
+    const uint32_t v = raw_buffer_load_u32(rsrc, byteOffset, scalarOffset, 0);
     return __half22float2(__builtin_bit_cast(__half2, v));
   } else {
-    const f32x2_t v = raw_buffer_load_f32x2(rsrc, byteOffset, 0, 0);
+    const f32x2_t v = raw_buffer_load_f32x2(rsrc, byteOffset, scalarOffset, 0);
     return make_float2(v[0], v[1]);
   }
 }
@@ -132,9 +134,14 @@ __device__ Contribution gather_generic(const FrArgs &a, __amdgpu_buffer_rsrc_t r
 }
 
 // Branch-free gather of one voxel-update over the NB x NB candidate box around (qx, qy).
-// Candidates the reference skips (outside the blob, outside the slice) read table[nTable] = 0
-// and an out-of-range pixel (= 0), adding exactly +0; the running sums never hold -0, so the
-// float results equal the reference's.  Returns false when the footprint is wider than NB.
+//
+// Exact strategy: candidates the reference skips (outside the blob, outside the slice) read
+// table[nTable] = 0 and an out-of-range pixel (= 0), adding exactly +0; the running sums never
+// hold -0, so the float results equal the reference's whatever the slice values are.
+// Fast-math strategy (finite slice values): the table index is clamped into the zero entry,
+// so out-of-blob candidates weigh 0 without a test; rows and columns outside the slice still
+// read 0 through the bounds-checked buffer.
+// Returns false when the footprint is wider than NB.
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds>
 __device__ __forceinline__ bool gather_box(const FrArgs &a, __amdgpu_buffer_rsrc_t rsrc,
                                            const float *tbl, float qx, float qy, float zSqr,
@@ -149,19 +156,22 @@ __device__ __forceinline__ bool gather_box(const FrArgs &a, __amdgpu_buffer_rsrc
   minY = minY < 0 ? 0 : minY;
   maxX = maxX > a.xSize - 1 ? a.xSize - 1 : maxX;
   maxY = maxY > a.ySize - 1 ? a.ySize - 1 : maxY;
-  const bool fits = (maxX - minX < kNB) && (maxY - minY < kNB);
+  const int nx = maxX - minX, ny = maxY - minY;
+  const bool fits = (nx < kNB) && (ny < kNB);
   constexpr int kPix = kHalf ? 4 : 8;
+  // (float)j == (float)minX + jj exactly for slice indices
+  const float fx0 = (float)minX, fy0 = (float)minY;
 
   float xd2[kNB], yz[kNB];
 #pragma unroll
   for (int jj = 0; jj < kNB; ++jj) {
-    float xD = qx - (float)(minX + jj);
-    xd2[jj] = (minX + jj <= maxX) ? xD * xD : inf;
+    const float xD = qx - (fx0 + (float)jj);
+    xd2[jj] = (jj <= nx) ? xD * xD : inf;
   }
 #pragma unroll
   for (int ii = 0; ii < kNB; ++ii) {
-    float dy = qy - (float)(minY + ii);
-    yz[ii] = (minY + ii <= maxY) ? dy * dy + zSqr : inf;
+    const float dy = qy - (fy0 + (float)ii);
+    yz[ii] = (ii <= ny) ? dy * dy + zSqr : inf;
   }
   const uint32_t rowStride = (uint32_t)(a.xSize * kPix);
   const uint32_t base = (uint32_t)(minY * a.xSize + minX) * kPix;
@@ -172,22 +182,34 @@ __device__ __forceinline__ bool gather_box(const FrArgs &a, __amdgpu_buffer_rsrc
 #pragma unroll
     for (int jj = 0; jj < kNB; ++jj) {
       const float d2 = xd2[jj] + yz[ii];
-      const bool in = d2 <= r2;
       float wb;
-      if constexpr (kMode == kModeLookup) {
-        int aux = kFastMath ? (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f)
-                            : (int)(d2 * c.cIDeltaSqrt + 0.5f);
-        if constexpr (kTableLds) {
-          wb = tbl[in ? aux : a.nTable];  // entry nTable of the LDS copy is 0
-        } else {
-          aux = aux > a.nTable - 1 ? a.nTable - 1 : aux;
-          wb = in ? tbl[aux] : 0.f;
-        }
+      uint32_t voff;
+      if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
+        // +inf and every d2 past the blob edge map to the zero entry
+        int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
+        aux = aux < a.nTable ? aux : a.nTable;
+        wb = tbl[aux];
+        voff = rowOff;
       } else {
-        wb = in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
+        const bool in = d2 <= r2;
+        if constexpr (kMode == kModeLookup) {
+          int aux = kFastMath ? (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f)
+                              : (int)(d2 * c.cIDeltaSqrt + 0.5f);
+          if constexpr (kTableLds) {
+            wb = tbl[in ? aux : a.nTable];  // entry nTable of the LDS copy is 0
+          } else {
+            aux = aux > a.nTable - 1 ? a.nTable - 1 : aux;
+            wb = in ? tbl[aux] : 0.f;
+          }
+        } else {
+          wb = in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
+        }
+        voff = in ? rowOff : kOutOfRange;
       }
       const float weight = kUnitWeight ? wb : wb * dw;
-      const float2 f = load_pixel<kHalf>(rsrc, (in ? rowOff : kOutOfRange) + jj * kPix);
+      // the column offset travels in the instruction's scalar offset; the range check on the
+      // vector offset alone still sends skipped candidates out of range
+      const float2 f = load_pixel<kHalf>(rsrc, voff, (uint32_t)(jj * kPix));
       w += weight;
       if constexpr (kFastMath) {
         vr = __builtin_fmaf(f.x, weight, vr);
@@ -236,11 +258,20 @@ __device__ __forceinline__ bool blob_predicate(const umpa_traverse_space &s, con
   return pass;
 }
 
+// Per-wave LDS state: running sums of the wave's 8 x 8 x 8 brick and the queue of voxels that
+// pass the slab test for the current space.
+struct WaveScratch {
+  float accR[kVox];
+  float accI[kVox];
+  float accW[kVox];
+  uint16_t queue[kVox];
+};
+
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount, bool kTableLds>
 __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
 {
   extern __shared__ float s_table[];
-  __shared__ float4 s_queue[kWaves][kQueue];
+  __shared__ WaveScratch s_wave[kWaves];
   if constexpr (kMode == kModeLookup && kTableLds) {
     for (int i = threadIdx.x; i < a.nTable; i += kThreads) s_table[i] = a.table[i];
     if (threadIdx.x == 0) s_table[a.nTable] = 0.f;
@@ -248,11 +279,16 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
   }
   const float *tbl = (kMode == kModeLookup && kTableLds) ? s_table : a.table;
   const int lane = threadIdx.x & 63;
-  float4 *queue = s_queue[threadIdx.x >> 6];
-  const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+  WaveScratch &ws = s_wave[threadIdx.x >> 6];
+  const int lx = lane & (kEdge - 1), ly = lane / kEdge;
   const umpa_fr_constants &c = a.c;
   const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
-  const float brickRadius = 4.0927375f;  // |(1.5, 1.5, 3.5)|, half diagonal of a brick
+  const float r = c.cBlobRadius;
+  const float r2 = r * r;
+  constexpr float kHalfEdge = 0.5f * (kEdge - 1);
+  // phase-1 filter on the squared slab coordinate: the reference's own test for blobs; for the
+  // nearest-neighbour traversal the visited voxel lies within 1.5 of the plane
+  const float slab2 = kMode == kModeNearest ? 2.56f : r2;
   constexpr int kPix = kHalf ? 4 : 8;
 
   for (;;) {
@@ -261,150 +297,181 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
     b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
     if (b >= a.nBricks) break;
     const int4 bo = reinterpret_cast<const int4 *>(a.bricks)[b];
+    // work item: brick 8 x 8 x depth at bo.xyz; heavy bricks run at raised wave priority; a
+    // brick split into `parts` slice ranges accumulates its range from zero and merges the
+    // partial sums with atomics (fast-math schedules only)
+    const int depth = bo.w & 15;
+    const bool heavy = (bo.w >> 4) & 1;
+    const int parts = (bo.w >> 8) & 255, part = (bo.w >> 16) & 255;
+    const bool split = parts > 1;
+    const int sBegin = split ? (int)((int64_t)a.nSpaces * part / parts) : 0;
+    const int sEnd = split ? (int)((int64_t)a.nSpaces * (part + 1) / parts) : a.nSpaces;
+    if (heavy) __builtin_amdgcn_s_setprio(2);
+    // the lane owns the z column (x, y, bo.z .. bo.z + depth - 1) of the brick
     const int x = bo.x + lx, y = bo.y + ly;
-    const int zz[2] = { bo.z + lz, bo.z + lz + 4 };
-    bool inside[2];
-    int64_t vidx[2];
-    float accR[2], accI[2], accW[2], pz[2], dsq[2];
+    const bool colIn = x < a.vx && y < a.vy;
+    const int nz = min(depth, a.vz - bo.z);
     const float px = (float)(x - cxi), py = (float)(y - cyi);
 #pragma unroll
-    for (int v = 0; v < 2; ++v) {
-      inside[v] = x < a.vx && y < a.vy && zz[v] < a.vz;
-      vidx[v] = ((int64_t)zz[v] * a.vy + y) * a.vx + x;
-      pz[v] = (float)(zz[v] - cyi);
-      dsq[v] = px * px + py * py + pz[v] * pz[v];
-      accR[v] = accI[v] = accW[v] = 0.f;
-      if (inside[v]) {
-        float2 t = reinterpret_cast<const float2 *>(a.volume)[vidx[v]];
-        accR[v] = t.x;
-        accI[v] = t.y;
-        accW[v] = a.weights[vidx[v]];
+    for (int k = 0; k < kEdge; ++k) {
+      const int li = k * 64 + lane;
+      float vr = 0.f, vi = 0.f, vw = 0.f;
+      if (!split && colIn && k < nz) {
+        const int64_t gi = ((int64_t)(bo.z + k) * a.vy + y) * a.vx + x;
+        const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
+        vr = t.x;
+        vi = t.y;
+        vw = a.weights[gi];
       }
+      ws.accR[li] = vr;
+      ws.accI[li] = vi;
+      ws.accW[li] = vw;
     }
+    wave_sync();
     unsigned long long nupd = 0;
-    // brick centre in centred volume coordinates
-    const float bcx = (float)(bo.x - cxi) + 1.5f;
-    const float bcy = (float)(bo.y - cyi) + 1.5f;
-    const float bcz = (float)(bo.z - cyi) + 3.5f;
+    // brick centre in centred volume coordinates, half extents (3.5, 3.5, (depth - 1) / 2)
+    const float hz = 0.5f * (float)(depth - 1);
+    const float bcx = (float)(bo.x - cxi) + kHalfEdge;
+    const float bcy = (float)(bo.y - cyi) + kHalfEdge;
+    const float bcz = (float)(bo.z - cyi) + hz;
     const float bnorm = sqrtf(bcx * bcx + bcy * bcy + bcz * bcz);
-    const float dmin = fmaxf(bnorm - brickRadius, 0.f);
+    const float dmin = fmaxf(bnorm - sqrtf(2.f * kHalfEdge * kHalfEdge + hz * hz), 0.f);
     const float eps = 0.02f + 1e-5f * bnorm;
 
-    for (int base = 0; base < a.nSpaces; base += 64) {
+    for (int base = sBegin; base < sEnd; base += 64) {
       const int si = base + lane;
       bool pass = false;
-      if (si < a.nSpaces) {
+      if (si < sEnd) {
         const umpa_traverse_space &sp = a.spaces[si];
         const float(*t)[3] = sp.transformInv;
-        float qz = t[2][0] * bcx + t[2][1] * bcy + t[2][2] * bcz;
-        float ez = 1.5f * (fabsf(t[2][0]) + fabsf(t[2][1])) + 3.5f * fabsf(t[2][2]);
-        float qx = t[0][0] * bcx + t[0][1] * bcy + t[0][2] * bcz;
-        float ex = 1.5f * (fabsf(t[0][0]) + fabsf(t[0][1])) + 3.5f * fabsf(t[0][2]);
+        const float qz = t[2][0] * bcx + t[2][1] * bcy + t[2][2] * bcz;
+        const float ez = kHalfEdge * (fabsf(t[2][0]) + fabsf(t[2][1])) + hz * fabsf(t[2][2]);
+        const float qx = t[0][0] * bcx + t[0][1] * bcy + t[0][2] * bcz;
+        const float ex = kHalfEdge * (fabsf(t[0][0]) + fabsf(t[0][1])) + hz * fabsf(t[0][2]);
         pass = fabsf(qz) <= a.cullHalfWidth + ez + eps && qx >= -a.cullHalfSpace - ex - eps &&
                dmin * dmin <= sp.maxDistanceSqr * 1.0001f + eps && sp.projectionIndex >= 0 &&
                sp.projectionIndex < a.nProj && sp.dir >= 0 && sp.dir <= 2;
       }
       unsigned long long mask = __ballot(pass);
       while (mask) {
-        const int k = __builtin_ctzll(mask);
+        const int k0 = __builtin_ctzll(mask);
         mask &= mask - 1;
-        // the space index is wave-uniform: say so, so its record is read with scalar loads and
-        // the slice's buffer descriptor lives in scalar registers
-        const int si_u = __builtin_amdgcn_readfirstlane(base + k);
-        const umpa_traverse_space sp = a.spaces[si_u];
+        // the space index is wave-uniform: say so, so its record is read once per wave and the
+        // slice's buffer descriptor lives in scalar registers
+        const umpa_traverse_space sp = a.spaces[__builtin_amdgcn_readfirstlane(base + k0)];
         const int proj = __builtin_amdgcn_readfirstlane(sp.projectionIndex);
         const int64_t off = (int64_t)proj * a.sliceElems * kPix;
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
           (void *)((const char *)a.fft + off), 0, (int)(a.sliceElems * kPix), kBufferConfig);
         const float scale = kHalf ? a.scales[proj] : 1.f;
+        const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
+                    t22 = sp.transformInv[2][2];
 
-        // 1. predicate for the lane's two voxels
-        bool pv[2];
-        float qx[2], qy[2], zs[2];
+        // phase 1: slab test of the column's 8 voxels.  qz = (t20 px + t21 py) + t22 pz is the
+        // reference's left-to-right sum, so the shared partial sum changes no bit.
+        const float cz = t20 * px + t21 * py;
+        unsigned long long m[kEdge];
+        int cnt = 0;
 #pragma unroll
-        for (int v = 0; v < 2; ++v) {
-          if constexpr (kMode == kModeNearest) {
-            qx[v] = sp.transformInv[0][0] * px + sp.transformInv[0][1] * py + sp.transformInv[0][2] * pz[v];
-            qy[v] = sp.transformInv[1][0] * px + sp.transformInv[1][1] * py + sp.transformInv[1][2] * pz[v];
-            zs[v] = 0.f;
-            pv[v] = inside[v] && !(dsq[v] > sp.maxDistanceSqr) && !(qx[v] < 0.f) &&
-                    visited_nearest(sp, x, y, zz[v]);
-          } else {
-            pv[v] = blob_predicate(sp, c, x, y, zz[v], px, py, pz[v], dsq[v], qx[v], qy[v], zs[v]);
-            pv[v] = pv[v] && inside[v];
-          }
+        for (int k = 0; k < kEdge; ++k) {
+          const float pz = (float)(bo.z + k - cyi);
+          const float qz = cz + t22 * pz;
+          const bool ok = colIn && k < nz && !(qz * qz > slab2);
+          m[k] = __ballot(ok);
         }
-        // 2. compaction into the wave's queue
-        const unsigned long long m0 = __ballot(pv[0]), m1 = __ballot(pv[1]);
-        const int c0 = __popcll(m0);
-        const int cnt = c0 + __popcll(m1);
+#pragma unroll
+        for (int k = 0; k < kEdge; ++k) {
+          if ((m[k] >> lane) & 1ull) ws.queue[cnt + (int)lanes_below(m[k])] = (uint16_t)(k * 64 + lane);
+          cnt += __popcll(m[k]);
+        }
         if (cnt == 0) continue;
-        const int r0 = (int)lanes_below(m0), r1 = c0 + (int)lanes_below(m1);
-        if (pv[0]) queue[r0] = make_float4(qx[0], qy[0], zs[0], 0.f);
-        if (pv[1]) queue[r1] = make_float4(qx[1], qy[1], zs[1], 0.f);
         wave_sync();
-        // 3. gather with full lanes
         const bool unit = sp.weight == 1.f;
+
+        // phase 2: the rest of the predicate and the gather, with full lanes
         for (int t0 = 0; t0 < cnt; t0 += 64) {
           const int item = t0 + lane;
           const bool act = item < cnt;
-          const float4 q = queue[act ? item : 0];
-          Contribution o;
+          const int v = act ? (int)ws.queue[item] : 0;
+          const int vl = v & 63, vk = v >> 6;
+          const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bo.z + vk;
+          const float pxv = (float)(xv - cxi), pyv = (float)(yv - cyi), pzv = (float)(zv - cyi);
+          const float dsq = pxv * pxv + pyv * pyv + pzv * pzv;
+          float qx, qy, zs;
+          bool hit;
           if constexpr (kMode == kModeNearest) {
-            // ProcessVoxel (fr_cpu_kernels.hpp:50-67): nearest pixel, weight = space weight
-            int ix = (int)(q.x + 0.5f);
-            ix = ix > a.xSize - 1 ? a.xSize - 1 : ix;
-            ix = ix < 0 ? 0 : ix;
-            int iy = (int)(q.y + 0.5f + (float)(c.cMaxVolumeIndexYZ / 2));
-            iy = iy > a.ySize - 1 ? a.ySize - 1 : iy;
-            iy = iy < 0 ? 0 : iy;
-            const float wgt = 1.f * sp.weight;
-            float2 f = load_pixel<kHalf>(rsrc, (uint32_t)((iy * a.xSize + ix) * kPix));
-            o.re = f.x * wgt;
-            o.im = f.y * wgt;
-            o.w = wgt;
+            qx = sp.transformInv[0][0] * pxv + sp.transformInv[0][1] * pyv + sp.transformInv[0][2] * pzv;
+            qy = sp.transformInv[1][0] * pxv + sp.transformInv[1][1] * pyv + sp.transformInv[1][2] * pzv;
+            zs = 0.f;
+            hit = act && !(dsq > sp.maxDistanceSqr) && !(qx < 0.f) && visited_nearest(sp, xv, yv, zv);
           } else {
-            bool fits;
-            if (unit) {
-              fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds>(
-                a, rsrc, tbl, q.x, q.y, q.z, sp.weight, o);
-            } else {
-              fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds>(
-                a, rsrc, tbl, q.x, q.y, q.z, sp.weight, o);
-            }
-            if (!fits) o = gather_generic<kMode, kOrder, kHalf, kFastMath>(a, rsrc, tbl, q.x, q.y, q.z, sp.weight);
-            if constexpr (kHalf) {
-              o.re *= scale;
-              o.im *= scale;
-            }
+            hit = act && blob_predicate(sp, c, xv, yv, zv, pxv, pyv, pzv, dsq, qx, qy, zs);
           }
-          wave_sync();
-          if (act) queue[item] = make_float4(o.re, o.im, o.w, 0.f);
+          if (hit) {
+            Contribution o;
+            if constexpr (kMode == kModeNearest) {
+              // ProcessVoxel (fr_cpu_kernels.hpp:50-67): nearest pixel, weight = space weight
+              int ix = (int)(qx + 0.5f);
+              ix = ix > a.xSize - 1 ? a.xSize - 1 : ix;
+              ix = ix < 0 ? 0 : ix;
+              int iy = (int)(qy + 0.5f + (float)(c.cMaxVolumeIndexYZ / 2));
+              iy = iy > a.ySize - 1 ? a.ySize - 1 : iy;
+              iy = iy < 0 ? 0 : iy;
+              const float wgt = 1.f * sp.weight;
+              float2 f = load_pixel<kHalf>(rsrc, (uint32_t)((iy * a.xSize + ix) * kPix));
+              if constexpr (kHalf) {
+                f.x *= scale;
+                f.y *= scale;
+              }
+              o.re = f.x * wgt;
+              o.im = f.y * wgt;
+              o.w = wgt;
+            } else {
+              bool fits;
+              if (unit) {
+                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds>(
+                  a, rsrc, tbl, qx, qy, zs, sp.weight, o);
+              } else {
+                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds>(
+                  a, rsrc, tbl, qx, qy, zs, sp.weight, o);
+              }
+              if (!fits)
+                o = gather_generic<kMode, kOrder, kHalf, kFastMath>(a, rsrc, tbl, qx, qy, zs, sp.weight);
+              if constexpr (kHalf) {
+                o.re *= scale;
+                o.im *= scale;
+              }
+            }
+            // volume += contribution, in space order (one item per voxel per space)
+            ws.accR[v] = ws.accR[v] + o.re;
+            ws.accI[v] = ws.accI[v] + o.im;
+            ws.accW[v] = ws.accW[v] + o.w;
+          }
+          if constexpr (kCount) nupd += (unsigned long long)__popcll(__ballot(hit));
         }
-        wave_sync();
-        // 4. owners add their contributions in space order
-        if (pv[0]) {
-          const float4 r = queue[r0];
-          accR[0] = accR[0] + r.x;
-          accI[0] = accI[0] + r.y;
-          accW[0] = accW[0] + r.z;
-        }
-        if (pv[1]) {
-          const float4 r = queue[r1];
-          accR[1] = accR[1] + r.x;
-          accI[1] = accI[1] + r.y;
-          accW[1] = accW[1] + r.z;
-        }
-        if constexpr (kCount) nupd += (unsigned long long)cnt;
         wave_sync();
       }
     }
-    for (int v = 0; v < 2; ++v) {
-      if (inside[v]) {
-        reinterpret_cast<float2 *>(a.volume)[vidx[v]] = make_float2(accR[v], accI[v]);
-        a.weights[vidx[v]] = accW[v];
+#pragma unroll
+    for (int k = 0; k < kEdge; ++k) {
+      if (colIn && k < nz) {
+        const int li = k * 64 + lane;
+        const int64_t gi = ((int64_t)(bo.z + k) * a.vy + y) * a.vx + x;
+        if (split) {
+          float *vol = reinterpret_cast<float *>(a.volume);
+          if (ws.accW[li] != 0.f || ws.accR[li] != 0.f || ws.accI[li] != 0.f) {
+            atomicAdd(vol + 2 * gi, ws.accR[li]);
+            atomicAdd(vol + 2 * gi + 1, ws.accI[li]);
+            atomicAdd(a.weights + gi, ws.accW[li]);
+          }
+        } else {
+          reinterpret_cast<float2 *>(a.volume)[gi] = make_float2(ws.accR[li], ws.accI[li]);
+          a.weights[gi] = ws.accW[li];
+        }
       }
     }
+    wave_sync();
+    if (heavy) __builtin_amdgcn_s_setprio(0);
     if constexpr (kCount) {
       if (lane == 0 && nupd) atomicAdd(a.updates, nupd);
     }
@@ -420,13 +487,16 @@ static hipError_t launch_one(const FrArgs &a, int numCUs, hipStream_t st)
   const bool lds = kMode == kModeLookup && a.nTable <= kMaxLdsTable;
   const size_t shmem = lds ? (size_t)(a.nTable + 1) * sizeof(float) : 0;
   int blocks = (a.nBricks + kWaves - 1) / kWaves;
-  int maxBlocks = numCUs * 2;
+  int maxBlocks = numCUs;  // the per-wave LDS state allows one workgroup per CU
   if (blocks > maxBlocks) blocks = maxBlocks;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(fr_reset_counter, dim3(1), dim3(1), 0, st, a.counter);
   if (lds) {
-    hipLaunchKernelGGL((fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true>),
-                       dim3(blocks), dim3(kThreads), shmem, st, a);
+    auto kern = fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), shmem, st, a);
   } else {
     hipLaunchKernelGGL((fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false>),
                        dim3(blocks), dim3(kThreads), 0, st, a);

@@ -28,7 +28,7 @@ struct FrArgs {
   int nBricks;
   int *counter;         // work counter, zeroed by the launch itself
   unsigned long long *updates;  // voxel-update counter (counting variant only)
-  float cullHalfWidth;  // slab half width used by the brick cull (blob radius, 1 for nearest)
+  float cullHalfWidth;  // slab half width used by the brick cull (blob radius, 1.6 for nearest)
   float cullHalfSpace;  // allowed q.x below zero (blob radius, 0 for nearest)
 };
 
