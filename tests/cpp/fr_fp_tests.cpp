@@ -1,0 +1,530 @@
+// The reference's FR and FP operation tests (tests/operations/fourier_reconstruction/
+// fr_tests.hpp + afr_common.hpp, tests/operations/fourier_processing/fp_tests.hpp +
+// afp_common.hpp), written against this library's C++ API with FRHIP / FPHIP as the backend and
+// managed memory as the payload storage (as the reference's CUDA test fixtures use).  gtest is
+// not available, so a minimal runner with the same assertion semantics is used
+// (ASSERT_FLOAT_EQ = within 4 ulps).
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "umpalumpa_amd/operations/fourier_processing/afp.hpp"
+#include "umpalumpa_amd/operations/fourier_reconstruction/afr.hpp"
+
+using namespace umpalumpa;
+using namespace umpalumpa::data;
+
+// ---- minimal test runner -----------------------------------------------------------------
+struct Failure {};
+static int g_failures = 0;
+static std::string g_current;
+#define ASSERT_TRUE(c)                                                                         \
+  do {                                                                                       \
+    if (!(c)) {                                                                              \
+      std::printf("  %s:%d: ASSERT_TRUE(%s) failed\n", __FILE__, __LINE__, #c);             \
+      throw Failure{};                                                                       \
+    }                                                                                        \
+  } while (0)
+#define ASSERT_EQ(a, b)                                                                        \
+  do {                                                                                       \
+    if (!((a) == (b))) {                                                                     \
+      std::printf("  %s:%d: ASSERT_EQ(%s, %s) failed\n", __FILE__, __LINE__, #a, #b);      \
+      throw Failure{};                                                                       \
+    }                                                                                        \
+  } while (0)
+static bool float_eq(float a, float b)
+{
+  if (a == b) return true;
+  if (std::isnan(a) || std::isnan(b)) return false;
+  int32_t ia, ib;
+  std::memcpy(&ia, &a, 4);
+  std::memcpy(&ib, &b, 4);
+  if ((ia < 0) != (ib < 0)) return false;
+  return std::abs((int64_t)ia - (int64_t)ib) <= 4;
+}
+#define ASSERT_FLOAT_EQ(a, b)                                                                  \
+  do {                                                                                       \
+    if (!float_eq((a), (b))) {                                                               \
+      std::printf("  %s:%d: ASSERT_FLOAT_EQ(%s, %s) failed: %.9g vs %.9g\n", __FILE__,      \
+                  __LINE__, #a, #b, (double)(a), (double)(b));                               \
+      throw Failure{};                                                                       \
+    }                                                                                        \
+  } while (0)
+#define ASSERT_NE(a, b)                                                                        \
+  do {                                                                                       \
+    if ((a) == (b)) {                                                                        \
+      std::printf("  %s:%d: ASSERT_NE(%s, %s) failed\n", __FILE__, __LINE__, #a, #b);      \
+      throw Failure{};                                                                       \
+    }                                                                                        \
+  } while (0)
+
+static std::vector<std::pair<std::string, std::function<void()>>> &registry()
+{
+  static std::vector<std::pair<std::string, std::function<void()>>> r;
+  return r;
+}
+struct Reg {
+  Reg(const char *n, std::function<void()> f) { registry().emplace_back(n, std::move(f)); }
+};
+#define TEST(name)                                                                             \
+  static void name();                                                                        \
+  static Reg reg_##name(#name, name);                                                        \
+  static void name()
+
+// ---- payload memory: managed (or plain host memory for host-only tests), zero-initialised,
+// freed with the fixture --------------------------------------------------------------------
+struct Managed {
+  explicit Managed(bool hostOnly = false) : host(hostOnly) {}
+  bool host;
+  std::vector<void *> blocks;
+  PhysicalDescriptor Create(size_t bytes, DataType type)
+  {
+    void *p = nullptr;
+    if (bytes) {
+      if (host) {
+        p = std::calloc(1, bytes);
+        if (!p) throw Failure{};
+      } else {
+        if (hipMallocManaged(&p, bytes) != hipSuccess) {
+          std::printf("  hipMallocManaged(%zu) failed\n", bytes);
+          throw Failure{};
+        }
+        std::memset(p, 0, bytes);
+      }
+      blocks.push_back(p);
+    }
+    return PhysicalDescriptor(p, bytes, type, ManagedBy::Manually, nullptr);
+  }
+  ~Managed()
+  {
+    if (host) {
+      for (void *p : blocks) std::free(p);
+      return;
+    }
+    (void)hipDeviceSynchronize();
+    for (void *p : blocks) (void)hipFree(p);
+  }
+};
+
+// ======================= FR (fr_tests.hpp / afr_common.hpp) ================================
+namespace fr {
+using namespace umpalumpa::fourier_reconstruction;
+
+struct Fixture {
+  Managed mem;
+  Payload<FourierDescriptor> fft, volume;
+  Payload<LogicalDescriptor> weight, space, table;
+
+  static Payload<FourierDescriptor> MakeFFT(Managed &m, const Size &s)
+  {
+    FourierDescriptor ld(s, PaddingDescriptor(), FourierDescriptor::FourierSpaceDescriptor{});
+    auto t = DataType::Get<std::complex<float>>();
+    return Payload(ld, m.Create(ld.Elems() * t.GetSize(), t), "fft");
+  }
+  static Payload<FourierDescriptor> MakeVolume(Managed &m, const Size &s)
+  {
+    FourierDescriptor::FourierSpaceDescriptor fd;
+    fd.hasSymetry = true;
+    FourierDescriptor ld(s.CopyFor(1), PaddingDescriptor(), fd);
+    auto t = DataType::Get<std::complex<float>>();
+    return Payload(ld, m.Create(ld.Elems() * t.GetSize(), t), "volume");
+  }
+  template<typename T> static Payload<LogicalDescriptor> MakeLogical(Managed &m, const Size &s)
+  {
+    LogicalDescriptor ld(s);
+    auto t = DataType::Get<T>();
+    return Payload(ld, m.Create(ld.Elems() * t.GetSize(), t), "");
+  }
+
+  Fixture(const Settings &s, const Size &projection)
+    : fft(MakeFFT(mem, projection)),
+      volume(MakeVolume(mem, Size(projection.y + 1, projection.y + 1, projection.y + 1, 1))),
+      weight(MakeLogical<float>(mem, Size(projection.y + 1, projection.y + 1, projection.y + 1, 1))),
+      space(MakeLogical<TraverseSpace>(mem, Size(1, 1, 1, 1))),
+      table(MakeLogical<float>(mem,
+        Size(s.GetInterpolation() == Settings::Interpolation::kLookup ? 10000 : 0, 1, 1, 1)))
+  {}
+};
+
+static void TestPoint(const umpa_point3 &l, float x, float y, float z)
+{
+  ASSERT_FLOAT_EQ(l.x, x);
+  ASSERT_FLOAT_EQ(l.y, y);
+  ASSERT_FLOAT_EQ(l.z, z);
+}
+
+// known answers of the reference (afr_common.hpp:136-254); dir: 0 XY, 1 XZ, 2 YZ
+struct KAT {
+  int minY, minX, minZ, maxY, maxX, maxZ;
+  float dist;
+  int dir;
+  float n[3], top[3], bottom[3];
+};
+static void TestSpace(const TraverseSpace &s, const KAT &k)
+{
+  ASSERT_EQ(s.minY, k.minY);
+  ASSERT_EQ(s.minX, k.minX);
+  ASSERT_EQ(s.minZ, k.minZ);
+  ASSERT_EQ(s.maxY, k.maxY);
+  ASSERT_EQ(s.maxX, k.maxX);
+  ASSERT_EQ(s.maxZ, k.maxZ);
+  ASSERT_FLOAT_EQ(s.maxDistanceSqr, k.dist);
+  ASSERT_EQ(s.dir, k.dir);
+  TestPoint(s.unitNormal, k.n[0], k.n[1], k.n[2]);
+  TestPoint(s.topOrigin, k.top[0], k.top[1], k.top[2]);
+  TestPoint(s.bottomOrigin, k.bottom[0], k.bottom[1], k.bottom[2]);
+  ASSERT_FLOAT_EQ(s.weight, 1.f);
+  ASSERT_EQ(s.projectionIndex, 0);
+}
+
+// the reference's result check: a voxel is non-zero exactly when it is within the frequency
+// sphere, on the image side and within the blob distance of the plane (afr_common.hpp:256-314)
+static void TestResult(const AFR::OutputData &d, const TraverseSpace &s, float maxDistance)
+{
+  const auto &vsize = d.GetVolume().info.GetSize();
+  auto *vol = reinterpret_cast<std::complex<float> *>(d.GetVolume().GetPtr());
+  auto *wt = reinterpret_cast<float *>(d.GetWeight().GetPtr());
+  for (int z = 0; z < (int)vsize.z; ++z)
+    for (int y = 0; y < (int)vsize.y; ++y)
+      for (int x = 0; x < (int)vsize.x; ++x) {
+        const float px = (float)(x - ((int)vsize.x - 1) / 2);
+        const float py = (float)(y - ((int)vsize.y - 1) / 2);
+        const float pz = (float)(z - ((int)vsize.z - 1) / 2);
+        const size_t i = ((size_t)z * vsize.y + y) * vsize.x + x;
+        bool zero;
+        if (px * px + py * py + pz * pz > s.maxDistanceSqr) {
+          zero = true;
+        } else {
+          const float qx = s.transformInv[0][0] * px + s.transformInv[0][1] * py + s.transformInv[0][2] * pz;
+          const float off = std::abs(s.unitNormal.x * px + s.unitNormal.y * py + s.unitNormal.z * pz);
+          zero = qx < -maxDistance || off > maxDistance;
+        }
+        if (zero) {
+          ASSERT_FLOAT_EQ(vol[i].real(), 0.f);
+          ASSERT_FLOAT_EQ(vol[i].imag(), 0.f);
+          ASSERT_FLOAT_EQ(wt[i], 0.f);
+        } else {
+          ASSERT_NE(vol[i].real(), 0.f);
+          ASSERT_NE(vol[i].imag(), 0.f);
+          ASSERT_NE(wt[i], 0.f);
+        }
+      }
+}
+
+static void RunPlane(const Settings &settings, const float t[3][3], bool constantInput,
+  const KAT &kat)
+{
+  Fixture fx(settings, Size(5, 6, 1, 1));
+  auto &space = *reinterpret_cast<TraverseSpace *>(fx.space.GetPtr());
+  // FillTraverseSpace: imgSizeX = transformationSize.y / 2 (afr_common.hpp:50-68)
+  ComputeTraverseSpace(6 / 2, 6, t, space, fx.volume.info.GetSize().x - 1,
+    fx.volume.info.GetSize().y - 1, settings.GetType() == Settings::Type::kFast,
+    settings.GetBlobRadius(), 1.f);
+  TestSpace(space, kat);
+  auto out = AFR::OutputData(fx.volume, fx.weight);
+  auto in = AFR::InputData(fx.fft, fx.volume, fx.weight, fx.space, fx.table);
+  if (settings.GetInterpolation() == Settings::Interpolation::kLookup) AFR::FillBlobTable(in, settings);
+  auto *f = reinterpret_cast<float *>(fx.fft.GetPtr());
+  const size_t n = fx.fft.info.Elems() * 2;
+  if (constantInput) {
+    std::fill(f, f + n, 1.f);
+  } else {
+    std::iota(f, f + n, 1.f);
+  }
+  FRHIP op;
+  ASSERT_TRUE(op.Init(out, in, settings));
+  ASSERT_TRUE(op.Execute(out, in));
+  op.Synchronize();
+  TestResult(out, space, settings.GetType() == Settings::Type::kFast ? 0 : settings.GetBlobRadius());
+}
+
+static Settings Make(Settings::Type t, Settings::Interpolation i)
+{
+  Settings s;
+  s.SetType(t);
+  s.SetInterpolation(i);
+  return s;
+}
+static const float kXY[3][3] = { { 1, 0, 0 }, { 0, 1, 0 }, { 0, 0, 1 } };
+static const float kYZ[3][3] = { { 0, 0, 1 }, { 0, 1, 0 }, { -1, 0, 0 } };
+static const float kXZ[3][3] = { { 0, -1, 0 }, { 0, 0, 1 }, { -1, 0, 0 } };
+static const KAT kXYFast{ 0, 3, 3, 6, 6, 3, 9.f, 0, { 0, 0, 1 }, { 6, 0, 3 }, { 6, 0, 3 } };
+static const KAT kXYPrecise{ 0, 1, 1, 6, 6, 5, 24.0100002f, 0, { 0, 0, 1 }, { 7.9f, -1.9f, 1.1f },
+  { 7.9f, -1.9f, 4.9f } };
+static const KAT kYZFast{ 0, 3, 0, 6, 3, 3, 9.f, 2, { 1, 0, 0 }, { 3, 0, 0 }, { 3, 0, 0 } };
+static const KAT kYZPrecise{ 0, 1, 0, 6, 5, 5, 24.0100002f, 2, { 1, 0, 0 }, { 1.1f, -1.9f, -1.9f },
+  { 4.9f, -1.9f, -1.9f } };
+static const KAT kXZFast{ 3, 0, 0, 3, 6, 3, 9.f, 1, { 0, 1, 0 }, { 6, 3, 0 }, { 6, 3, 0 } };
+static const KAT kXZPrecise{ 1, 0, 0, 5, 6, 5, 24.0100002f, 1, { 0, 1, 0 }, { 7.9f, 1.1f, -1.9f },
+  { 7.9f, 4.9f, -1.9f } };
+
+using T = Settings::Type;
+using I = Settings::Interpolation;
+TEST(XYPlanePreciseDynamic) { RunPlane(Make(T::kPrecise, I::kDynamic), kXY, true, kXYPrecise); }
+TEST(XYPlaneFastDynamic) { RunPlane(Make(T::kFast, I::kDynamic), kXY, true, kXYFast); }
+TEST(XYPlanePreciseLookup) { RunPlane(Make(T::kPrecise, I::kLookup), kXY, true, kXYPrecise); }
+TEST(XYPlaneFastLookup) { RunPlane(Make(T::kFast, I::kLookup), kXY, true, kXYFast); }
+TEST(YZPlanePreciseDynamic) { RunPlane(Make(T::kPrecise, I::kDynamic), kYZ, false, kYZPrecise); }
+TEST(YZPlaneFastDynamic) { RunPlane(Make(T::kFast, I::kDynamic), kYZ, false, kYZFast); }
+TEST(YZPlanePreciseLookup) { RunPlane(Make(T::kPrecise, I::kLookup), kYZ, false, kYZPrecise); }
+TEST(YZPlaneFastLookup) { RunPlane(Make(T::kFast, I::kLookup), kYZ, false, kYZFast); }
+TEST(XZPlanePreciseDynamic) { RunPlane(Make(T::kPrecise, I::kDynamic), kXZ, false, kXZPrecise); }
+TEST(XZPlaneFastDynamic) { RunPlane(Make(T::kFast, I::kDynamic), kXZ, false, kXZFast); }
+TEST(XZPlanePreciseLookup) { RunPlane(Make(T::kPrecise, I::kLookup), kXZ, false, kXZPrecise); }
+TEST(XZPlaneFastLookup) { RunPlane(Make(T::kFast, I::kLookup), kXZ, false, kXZFast); }
+
+// traverse-space known answers alone: host code, no device needed (afr_common.hpp:136-254)
+static void HostSpace(const float t[3][3], bool fast, const KAT &kat)
+{
+  TraverseSpace space{};
+  ComputeTraverseSpace(6 / 2, 6, t, space, 6, 6, fast, 1.9f, 1.f);
+  TestSpace(space, kat);
+}
+TEST(HostTraverseSpaceXY)
+{
+  HostSpace(kXY, true, kXYFast);
+  HostSpace(kXY, false, kXYPrecise);
+}
+TEST(HostTraverseSpaceYZ)
+{
+  HostSpace(kYZ, true, kYZFast);
+  HostSpace(kYZ, false, kYZPrecise);
+}
+TEST(HostTraverseSpaceXZ)
+{
+  HostSpace(kXZ, true, kXZFast);
+  HostSpace(kXZ, false, kXZPrecise);
+}
+
+// payload bookkeeping of the operation API, host only: sizes, validity and the equivalence
+// rules Execute applies (payload.hpp:36-79, payload_wrapper.hpp:54-172)
+TEST(HostPayloadContract)
+{
+  Settings s = Make(T::kPrecise, I::kLookup);
+  Managed m(true);
+  auto fft = Fixture::MakeFFT(m, Size(5, 6, 1, 3));
+  auto vol = Fixture::MakeVolume(m, Size(7, 7, 7, 1));
+  auto wt = Fixture::MakeLogical<float>(m, Size(7, 7, 7, 1));
+  auto sp = Fixture::MakeLogical<TraverseSpace>(m, Size(3, 1, 1, 1));
+  auto tb = Fixture::MakeLogical<float>(m, Size(10000, 1, 1, 1));
+  // frequency-domain payload of a 5 x 6 spatial image: 3 x 6 complex per slice
+  ASSERT_EQ(fft.GetRequiredBytes(), (size_t)3 * 6 * 3 * 8);
+  ASSERT_TRUE(fft.IsValid() && vol.IsValid() && wt.IsValid() && sp.IsValid() && tb.IsValid());
+  auto in = AFR::InputData(fft, vol, wt, sp, tb);
+  ASSERT_TRUE(in.IsValid());
+  // fewer slices than at Init: equivalent; more: not (size.hpp:62-65)
+  auto fewer = Fixture::MakeFFT(m, Size(5, 6, 1, 2));
+  auto more = Fixture::MakeFFT(m, Size(5, 6, 1, 4));
+  auto inFewer = AFR::InputData(fewer, vol, wt, sp, tb);
+  auto inMore = AFR::InputData(more, vol, wt, sp, tb);
+  ASSERT_TRUE(inFewer.IsEquivalentTo(in));
+  ASSERT_TRUE(!inMore.IsEquivalentTo(in));
+  // only the first two payloads take part in the equivalence (payload_wrapper.hpp:160-162)
+  auto sp2 = Fixture::MakeLogical<TraverseSpace>(m, Size(9, 1, 1, 1));
+  auto inSp = AFR::InputData(fft, vol, wt, sp2, tb);
+  ASSERT_TRUE(inSp.IsEquivalentTo(in));
+  // a payload whose memory is too small is invalid
+  LogicalDescriptor ld(Size(7, 7, 7, 1));
+  Payload<LogicalDescriptor> small(ld, m.Create(16, DataType::Get<float>()));
+  ASSERT_TRUE(!small.IsValid());
+  // constants and blob table through the API match the closed forms they restate
+  auto c = AFR::CreateConstants(in, s);
+  ASSERT_EQ(c.cMaxVolumeIndexX, 6);
+  ASSERT_EQ(c.cMaxVolumeIndexYZ, 6);
+  ASSERT_FLOAT_EQ(c.cBlobRadius, 1.9f);
+  ASSERT_FLOAT_EQ(c.cIDeltaSqrt, 9999.f / (1.9f * 1.9f));
+  AFR::FillBlobTable(in, s);
+  const float *t = reinterpret_cast<const float *>(tb.GetPtr());
+  // table[i] = kaiser(r_i) * iw0 and kaiser(0) = 1 (afr.cpp:22-34, kaiser.hpp)
+  ASSERT_FLOAT_EQ(t[0], c.cIw0);
+  ASSERT_TRUE(t[9999] >= 0.f && t[9999] < 1e-3f);
+  for (int i = 1; i < 10000; ++i) ASSERT_TRUE(t[i] <= t[i - 1]);
+}
+
+// operation contract: Execute before Init, odd volumes only, kLookup needs a table
+TEST(FRContract)
+{
+  Settings s = Make(T::kPrecise, I::kLookup);
+  Fixture fx(s, Size(5, 6, 1, 1));
+  auto out = AFR::OutputData(fx.volume, fx.weight);
+  auto in = AFR::InputData(fx.fft, fx.volume, fx.weight, fx.space, fx.table);
+  FRHIP op;
+  ASSERT_TRUE(!op.Execute(out, in));
+  ASSERT_TRUE(op.Init(out, in, s));
+  // a volume payload different from the output one is invalid (afr.hpp:50-52)
+  Managed m;
+  auto other = Fixture::MakeVolume(m, Size(7, 7, 7, 1));
+  auto out2 = AFR::OutputData(other, fx.weight);
+  ASSERT_TRUE(!op.Init(out2, in, s));
+  // even-sized volumes are rejected at Init (fr_cpu.cpp:18)
+  auto even = Fixture::MakeVolume(m, Size(8, 8, 8, 1));
+  auto evenW = Fixture::MakeLogical<float>(m, Size(8, 8, 8, 1));
+  auto out3 = AFR::OutputData(even, evenW);
+  auto in3 = AFR::InputData(fx.fft, even, evenW, fx.space, fx.table);
+  ASSERT_TRUE(!op.Init(out3, in3, s));
+  // kLookup without a table: Execute returns false (fr_cpu.cpp:27-29)
+  Settings d = Make(T::kPrecise, I::kDynamic);
+  Fixture fd(d, Size(5, 6, 1, 1));
+  auto out4 = AFR::OutputData(fd.volume, fd.weight);
+  auto in4 = AFR::InputData(fd.fft, fd.volume, fd.weight, fd.space, fd.table);
+  ASSERT_TRUE(op.Init(out4, in4, s));
+  ASSERT_TRUE(!op.Execute(out4, in4));
+  ASSERT_EQ(op.LastStatus(), UMPA_ERR_REJECTED);
+}
+}  // namespace fr
+
+// ======================= FP (fp_tests.hpp / afp_common.hpp) ================================
+namespace fp {
+using namespace umpalumpa::fourier_processing;
+using fourier_transformation::Locality;
+
+struct Fixture {
+  Managed mem;
+  Payload<FourierDescriptor> in, out;
+  Payload<LogicalDescriptor> filter;
+  static Payload<FourierDescriptor> Make(Managed &m, const Size &s)
+  {
+    FourierDescriptor ld(s, PaddingDescriptor(), FourierDescriptor::FourierSpaceDescriptor{});
+    auto t = DataType::Get<std::complex<float>>();
+    return Payload(ld, m.Create(ld.Elems() * t.GetSize(), t), "");
+  }
+  Fixture(const Size &sIn, const Size &sOut)
+    : in(Make(mem, sIn)), out(Make(mem, sOut)),
+      filter(LogicalDescriptor(sOut), mem.Create(LogicalDescriptor(sOut).Elems() * 4, DataType::Get<float>()))
+  {}
+};
+
+// expected output restated as the reference test does (afp_common.hpp:132-178)
+static void Check(const std::complex<float> *output, const Size &outSize,
+  const std::complex<float> *input, const Size &inSize, const Size &inSpatialSize,
+  float normFactor, const float *filter, const Settings &s)
+{
+  for (size_t n = 0; n < outSize.n; n++) {
+    size_t cropIndex = outSize.y / 2 + 1;
+    for (size_t y = 0; y < outSize.y; y++)
+      for (size_t x = 0; x < outSize.x; x++) {
+        auto inY = y < cropIndex ? y : inSize.y - outSize.y + y;
+        auto inIndex = n * inSize.single + inY * inSize.x + x;
+        size_t shiftedY = (y + outSize.y + outSize.y / 2) % outSize.y;
+        auto outIndex = n * outSize.single + (s.GetShift() ? shiftedY : y) * outSize.x + x;
+        float re = input[inIndex].real(), im = input[inIndex].imag();
+        if (s.GetApplyFilter()) {
+          re *= filter[y * outSize.x + x];
+          im *= filter[y * outSize.x + x];
+        }
+        if (s.GetNormalize()) {
+          re *= normFactor;
+          im *= normFactor;
+        }
+        if (s.GetCenter()) {
+          float cc = (float)(1 - 2 * ((int)(x + y) & 1));
+          re *= cc;
+          im *= cc;
+        }
+        if (s.GetMaxFreq().has_value()) {
+          auto freq = [](float i, float max) { return ((i <= (max / 2)) ? i : (i - max)) / max; };
+          auto fx = freq((float)x, (float)inSpatialSize.x), fy = freq((float)y, (float)inSpatialSize.y);
+          if (fx * fx + fy * fy > s.GetMaxFreq().value()) re = im = 0;
+        }
+        ASSERT_FLOAT_EQ(re, output[outIndex].real());
+        ASSERT_FLOAT_EQ(im, output[outIndex].imag());
+      }
+  }
+}
+
+static void RunFP(const Settings &settings, const Size &sIn, const Size &sOut)
+{
+  Fixture fx(sIn, sOut);
+  auto out = AFP::OutputData(fx.out);
+  auto in = AFP::InputData(fx.in, fx.filter);
+  auto *input = reinterpret_cast<std::complex<float> *>(fx.in.GetPtr());
+  for (size_t i = 0; i < fx.in.info.GetSize().total; i++)
+    input[i] = { static_cast<float>(i), static_cast<float>(i) };
+  auto *filter = reinterpret_cast<float *>(fx.filter.GetPtr());
+  if (settings.GetApplyFilter()) {
+    for (size_t i = 0; i < fx.filter.info.GetSize().total; i += 2) filter[i] = -1.0f;
+    for (size_t i = 1; i < fx.filter.info.GetSize().total; i += 2) filter[i] = 0.5f;
+  }
+  FPHIP op;
+  ASSERT_TRUE(op.Init(out, in, settings));
+  ASSERT_TRUE(op.Execute(out, in));
+  op.Synchronize();
+  float normFactor = 1.f / fx.in.info.GetSpatialSize().single;
+  Check(reinterpret_cast<std::complex<float> *>(fx.out.GetPtr()), fx.out.info.GetSize(), input,
+    fx.in.info.GetSize(), fx.in.info.GetSpatialSize(), normFactor, filter, settings);
+}
+
+TEST(ImageCropping) { RunFP(Settings(Locality::kOutOfPlace), Size(10, 10, 1, 1), Size(5, 5, 1, 1)); }
+TEST(ImageNoCropping) { RunFP(Settings(Locality::kOutOfPlace), Size(10, 10, 1, 1), Size(10, 10, 1, 1)); }
+TEST(ImageNormalize)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetNormalize(true);
+  RunFP(s, Size(10, 10, 1, 1), Size(10, 10, 1, 1));
+}
+TEST(ImageCentering)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetCenter(true);
+  RunFP(s, Size(12, 12, 1, 1), Size(12, 12, 1, 1));
+}
+TEST(ImageCropNormalizeCenter)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetNormalize(true);
+  s.SetCenter(true);
+  RunFP(s, Size(40, 40, 1, 5), Size(20, 20, 1, 5));
+}
+TEST(Filtering)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetApplyFilter(true);
+  RunFP(s, Size(12, 12, 1, 1), Size(12, 12, 1, 1));
+}
+TEST(MaxFreq)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetMaxFreq(0.25);
+  RunFP(s, Size(12, 12, 1, 1), Size(12, 12, 1, 1));
+}
+TEST(ShiftEven)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetShift(true);
+  RunFP(s, Size(6, 4, 1, 1), Size(6, 4, 1, 1));
+}
+TEST(ShiftOdd)
+{
+  Settings s(Locality::kOutOfPlace);
+  s.SetShift(true);
+  RunFP(s, Size(4, 3, 1, 1), Size(4, 3, 1, 1));
+}
+}  // namespace fp
+
+int main(int argc, char **argv)
+{
+  int run = 0;
+  for (auto &[name, fn] : registry()) {
+    if (argc > 1 && name.find(argv[1]) == std::string::npos) continue;
+    ++run;
+    try {
+      fn();
+      std::printf("[ OK ] %s\n", name.c_str());
+    } catch (const Failure &) {
+      std::printf("[FAIL] %s\n", name.c_str());
+      ++g_failures;
+    }
+    std::fflush(stdout);
+  }
+  std::printf("%d tests, %d failures\n", run, g_failures);
+  return g_failures ? 1 : 0;
+}
