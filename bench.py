@@ -41,7 +41,20 @@ def parse():
     p.add_argument("--strategy", default="exact", choices=["exact", "fast", "f16"])
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-slices", type=int, default=1000)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a plain 256 MiB device-to-device copy, 5 times: the known-size "
+                        "kernel that scales rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes")
     return p.parse_args()
+
+
+def copy_calibration(torch):
+    n = 256 << 20
+    src = torch.ones(n // 4, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "launches": 5}))
 
 
 def make_slices(torch, box, n, seed, dev):
@@ -106,6 +119,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     from umpalumpa_amd import fr as F
 
     dev = torch.device("cuda:0")

@@ -145,6 +145,62 @@ void umpa_fr_destroy(umpa_fr_op *op);
 int umpa_fr_set_count_updates(umpa_fr_op *op, int enable);
 int umpa_fr_last_updates(umpa_fr_op *op, uint64_t *updates);
 
+/* ---- Batched FR: many reconstructions through one persistent kernel ---------------------
+ * No counterpart in the reference API, which runs one AFR per volume (fr.cpp:9-76); a batch
+ * is the same AFR::Execute applied to several (slices, volume) jobs, each job's voxels
+ * receiving its spaces in index order as FRCPU would (results identical to running the jobs
+ * one by one with the same strategy; bit-identical to FRCPU under UMPA_FR_EXACT). */
+
+/* One reconstruction of a batch.  desc/settings as for umpa_fr_init (desc.fft.n = slices in
+ * `fft`); spaces: desc.traverseSpaces records, projectionIndex < desc.fft.n; fft complex
+ * float, or complex half when `scales` (one float per slice) is not NULL; all pointers device
+ * memory; table: device blob table of desc.blobTable floats (kLookup). */
+typedef struct umpa_fr_job {
+  umpa_fr_desc desc;
+  umpa_fr_settings settings;
+  const void *fft;
+  const float *scales;
+  void *volume;
+  float *weights;
+  const umpa_traverse_space *spaces;
+  const float *table;
+} umpa_fr_job;
+
+typedef struct umpa_fr_batch umpa_fr_batch;
+
+/* new batch runner on `device` and `stream` (NULL: the runner creates its own stream, which a
+ * graph capture needs) */
+int umpa_fr_batch_create(umpa_fr_batch **b, int device, void *stream);
+
+/* Plans the chunk loop of `nJobs` jobs: step c inserts spaces [c * chunkSpaces,
+ * (c + 1) * chunkSpaces) of every job that has them, one persistent launch per group of jobs
+ * sharing settings, table and slice precision; work items (bricks of all jobs of the step)
+ * are ordered heaviest first.  With `useGraph`, the whole loop is captured once as a HIP graph
+ * and replayed by each umpa_fr_batch_execute.  The job array is copied; the buffers it names
+ * must stay valid while the plan is used. */
+int umpa_fr_batch_plan(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                       uint32_t chunkSpaces, int strategy, int useGraph);
+
+/* Enqueues every step of the plan (asynchronous; volumes accumulate in place). */
+int umpa_fr_batch_execute(umpa_fr_batch *b);
+
+/* Enqueues one step over explicit space ranges: job i inserts spaces [begin[i], end[i]).  For
+ * streaming callers whose jobs advance independently (continuous batching); staging buffers
+ * are pinned and recycled by stream order, so successive calls do not block. */
+int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t *begin,
+                       const uint64_t *end, uint32_t nJobs, int strategy);
+
+/* number of steps of the current plan */
+int umpa_fr_batch_num_steps(umpa_fr_batch *b, uint32_t *steps);
+/* counting variant for the next execute / step (bypasses the graph); result after it */
+int umpa_fr_batch_set_count_updates(umpa_fr_batch *b, int enable);
+int umpa_fr_batch_last_updates(umpa_fr_batch *b, uint64_t *updates);
+/* records an event on the batch stream after the last enqueued work; query / wait on it */
+int umpa_fr_batch_record(umpa_fr_batch *b, uint64_t *ticket);
+int umpa_fr_batch_query(umpa_fr_batch *b, uint64_t ticket, int *done);
+int umpa_fr_batch_synchronize(umpa_fr_batch *b);
+void umpa_fr_batch_destroy(umpa_fr_batch *b);
+
 /* ---- FR host-side helpers (no GPU needed) ----------------------------------------------- */
 
 /* AFR::CreateConstants (afr.cpp:6-19) */
@@ -160,6 +216,17 @@ int umpa_fr_compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY,
                                    const float transform[9], umpa_traverse_space *space,
                                    uint32_t maxVolumeIndexX, uint32_t maxVolumeIndexYZ,
                                    int useFast, float blobRadius, float weight);
+
+/* GenerateTraverseSpaces (examples/fourier_reconstruction/fr.cpp:278-312) for n slices of a
+ * padded box P: record i * nSym + q gets transform R_i S_q (rotations: n row-major 3x3
+ * doubles, symmetries: nSym row-major 3x3 doubles or NULL for the identity; the product is
+ * formed in FP64 and rounded once to float), imgSizeX = P / 2, imgSizeY = P, maximum volume
+ * index P (afr_common.hpp:50-68) and projectionIndex = firstIndex + i. */
+int umpa_fr_generate_traverse_spaces(const double *rotations, uint64_t n,
+                                     const double *symmetries, uint32_t nSym,
+                                     uint32_t paddedBox, const umpa_fr_settings *settings,
+                                     float weight, int32_t firstIndex,
+                                     umpa_traverse_space *out);
 
 /* ---- FP: the AFP operation (afp.hpp:13-53), crop/filter/normalize/center/shift ----------- */
 

@@ -360,9 +360,10 @@ TEST(FRContract)
   FRHIP op;
   ASSERT_TRUE(!op.Execute(out, in));
   ASSERT_TRUE(op.Init(out, in, s));
-  // a volume payload different from the output one is invalid (afr.hpp:50-52)
+  // the output volume must describe the input one (afr.hpp:50-52); Init compares the
+  // data-less copies (basic_operation.hpp:33-39), so only descriptors can differ here
   Managed m;
-  auto other = Fixture::MakeVolume(m, Size(7, 7, 7, 1));
+  auto other = Fixture::MakeVolume(m, Size(9, 9, 9, 1));
   auto out2 = AFR::OutputData(other, fx.weight);
   ASSERT_TRUE(!op.Init(out2, in, s));
   // even-sized volumes are rejected at Init (fr_cpu.cpp:18)

@@ -64,6 +64,13 @@ class FRDesc(C.Structure):
                 ("blobTable", C.c_uint64)]
 
 
+class FRJob(C.Structure):
+    """umpa_fr_job: one reconstruction of a batch (device pointers)."""
+    _fields_ = [("desc", FRDesc), ("settings", FRSettings), ("fft", C.c_void_p),
+                ("scales", C.c_void_p), ("volume", C.c_void_p), ("weights", C.c_void_p),
+                ("spaces", C.c_void_p), ("table", C.c_void_p)]
+
+
 class FPSettings(C.Structure):
     _fields_ = [("center", C.c_int32), ("normalize", C.c_int32), ("applyFilter", C.c_int32),
                 ("shift", C.c_int32), ("hasMaxFreq", C.c_int32), ("maxFreq", C.c_float)]
@@ -96,6 +103,22 @@ PROTOTYPES = [
     ("umpa_fr_fill_blob_table", C.c_int, [_P, _U64, C.POINTER(FRDesc), C.POINTER(FRSettings)]),
     ("umpa_fr_compute_traverse_space", C.c_int, [C.c_uint32, C.c_uint32, _P, _P, C.c_uint32,
                                                 C.c_uint32, C.c_int, C.c_float, C.c_float]),
+    ("umpa_fr_generate_traverse_spaces", C.c_int, [_P, _U64, _P, C.c_uint32, C.c_uint32,
+                                                   C.POINTER(FRSettings), C.c_float, C.c_int32,
+                                                   _P]),
+    ("umpa_fr_batch_create", C.c_int, [C.POINTER(_P), C.c_int, _P]),
+    ("umpa_fr_batch_plan", C.c_int, [_P, C.POINTER(FRJob), C.c_uint32, C.c_uint32, C.c_int,
+                                     C.c_int]),
+    ("umpa_fr_batch_execute", C.c_int, [_P]),
+    ("umpa_fr_batch_step", C.c_int, [_P, C.POINTER(FRJob), C.POINTER(_U64), C.POINTER(_U64),
+                                     C.c_uint32, C.c_int]),
+    ("umpa_fr_batch_num_steps", C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    ("umpa_fr_batch_set_count_updates", C.c_int, [_P, C.c_int]),
+    ("umpa_fr_batch_last_updates", C.c_int, [_P, C.POINTER(_U64)]),
+    ("umpa_fr_batch_record", C.c_int, [_P, C.POINTER(_U64)]),
+    ("umpa_fr_batch_query", C.c_int, [_P, _U64, C.POINTER(C.c_int)]),
+    ("umpa_fr_batch_synchronize", C.c_int, [_P]),
+    ("umpa_fr_batch_destroy", None, [_P]),
     ("umpa_fp_create", C.c_int, [C.POINTER(_P), C.c_int, _P]),
     ("umpa_fp_init", C.c_int, [_P, C.POINTER(FPDesc), C.POINTER(FPSettings)]),
     ("umpa_fp_execute", C.c_int, [_P, _P, _U64, _P, _U64, _P]),

@@ -230,7 +230,7 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
   DeviceGuard g(op->device);
   if (!g.ok) return UMPA_ERR_DEVICE;
 
-  umpa::FrArgs a{};
+  umpa::FrJob a{};
   a.volume = volume;
   a.weights = weights;
   a.fft = fft;
@@ -240,22 +240,25 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
   a.sliceElems = (int64_t)op->desc.fft.x * (int64_t)op->desc.fft.y;
   a.nProj = (int)fftCount;
   a.nSpaces = (int)nSpaces;
+  a.spaceBegin = 0;
+  a.spaceEnd = (int)nSpaces;
   a.spaces = spaces;
-  a.table = table;
-  a.nTable = (int)op->desc.blobTable;
   a.c = op->constants;
   a.vx = (int)op->desc.volume.x;
   a.vy = (int)op->desc.volume.y;
   a.vz = (int)op->desc.volume.z;
-  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
-  const bool fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
-  a.bricks = fastMath ? op->dBricksSplit : op->dBricks;
-  a.nBricks = fastMath ? op->nBricksSplit : op->nBricks;
-  a.counter = op->dCounter;
-  a.updates = op->dUpdates;
   const bool nearest = op->settings.type == UMPA_FR_FAST;
   a.cullHalfWidth = nearest ? 1.6f : op->settings.blobRadius;
   a.cullHalfSpace = nearest ? 0.f : op->settings.blobRadius;
+  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
+  const bool fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
+  umpa::FrLaunch l{};
+  l.items = fastMath ? op->dBricksSplit : op->dBricks;
+  l.nItems = fastMath ? op->nBricksSplit : op->nBricks;
+  l.counter = op->dCounter;
+  l.updates = op->dUpdates;
+  l.table = table;
+  l.nTable = (int)op->desc.blobTable;
 
   umpa::FrVariant v{};
   if (nearest) {
@@ -269,12 +272,14 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
   v.order = op->settings.order;
   v.nb = op->nb;
   v.half = half;
-  v.fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
+  v.fastMath = fastMath;
   v.count = op->countUpdates;
+  v.batched = false;
   if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
   hipError_t e = hipSuccess;
   if (v.count) e = hipMemsetAsync(op->dUpdates, 0, sizeof(unsigned long long), op->stream);
-  if (e == hipSuccess) e = umpa::launch_fr_insert(a, v, op->numCUs, op->stream);
+  if (e == hipSuccess) e = umpa::prepare_fr_insert(v, l.nTable);
+  if (e == hipSuccess) e = umpa::launch_fr_insert(a, l, v, op->numCUs, op->stream);
   if (e == hipSuccess && v.count) {
     e = hipMemcpyAsync(&op->lastUpdates, op->dUpdates, sizeof(unsigned long long),
                        hipMemcpyDeviceToHost, op->stream);
@@ -370,6 +375,34 @@ int umpa_fr_compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY,
   if (!transform || !space) return UMPA_ERR_INVALID_ARG;
   umpa::host::compute_traverse_space(imgSizeX, imgSizeY, transform, *space, maxVolumeIndexX,
                                      maxVolumeIndexYZ, useFast != 0, blobRadius, weight);
+  return UMPA_OK;
+}
+
+int umpa_fr_generate_traverse_spaces(const double *rotations, uint64_t n,
+                                     const double *symmetries, uint32_t nSym,
+                                     uint32_t paddedBox, const umpa_fr_settings *s, float weight,
+                                     int32_t firstIndex, umpa_traverse_space *out)
+{
+  if ((!rotations && n) || !s || (!out && n) || paddedBox < 2) return UMPA_ERR_INVALID_ARG;
+  if (symmetries && nSym == 0) return UMPA_ERR_INVALID_ARG;
+  const double identity[9] = { 1, 0, 0, 0, 1, 0, 0, 0, 1 };
+  const double *sym = symmetries ? symmetries : identity;
+  const uint32_t k = symmetries ? nSym : 1;
+  for (uint64_t i = 0; i < n; ++i) {
+    const double *r = rotations + 9 * i;
+    for (uint32_t q = 0; q < k; ++q) {
+      const double *o = sym + 9 * q;
+      // transform = R_i S_q in FP64, rounded once to float
+      float t[9];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          t[3 * a + b] = (float)(r[3 * a] * o[b] + r[3 * a + 1] * o[3 + b] + r[3 * a + 2] * o[6 + b]);
+      umpa_traverse_space &rec = out[i * k + q];
+      umpa::host::compute_traverse_space(paddedBox / 2, paddedBox, t, rec, paddedBox, paddedBox,
+                                         s->type == UMPA_FR_FAST, s->blobRadius, weight);
+      rec.projectionIndex = firstIndex + (int32_t)i;
+    }
+  }
   return UMPA_OK;
 }
 

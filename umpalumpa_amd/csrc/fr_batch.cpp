@@ -1,0 +1,563 @@
+// Batched FR: several reconstructions inserted by one persistent launch per step.
+//
+// A step inserts a range of traverse spaces of every job it names.  Jobs that share a kernel
+// variant (settings, blob table, slice precision) go into one launch whose work items are the
+// bricks of all of them, heaviest first, so small and large volumes fill the GPU together.
+// Voxels still receive each job's spaces in index order, so a job's result does not depend on
+// the other jobs of the batch.
+//
+//   umpa_fr_batch_plan + execute: a static chunk loop (step c = spaces [c*C, (c+1)*C) of every
+//     job), all device-side state built once, the loop captured as one HIP graph;
+//   umpa_fr_batch_step: one step over explicit ranges, for streaming callers; job records and
+//     work items travel through a ring of pinned staging slots recycled in stream order.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <tuple>
+#include <utility>
+#include <vector>
+
+#include "fr_host.hpp"
+#include "launch.hpp"
+#include "umpalumpa_amd.h"
+
+namespace {
+
+int from_hip(hipError_t e)
+{
+  if (e == hipSuccess) return UMPA_OK;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return UMPA_ERR_OUT_OF_MEMORY;
+  return UMPA_ERR_DEVICE;
+}
+
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = false;
+  explicit DeviceGuard(int device)
+  {
+    if (hipGetDevice(&prev) != hipSuccess) return;
+    ok = (prev == device) || hipSetDevice(device) == hipSuccess;
+  }
+  ~DeviceGuard()
+  {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+bool size_valid(const umpa_size &s)
+{
+  if (s.n < 1) return false;
+  const bool d3 = s.z >= 2 && s.y >= 2 && s.x >= 2;
+  const bool d2 = s.z == 1 && s.y >= 2 && s.x >= 2;
+  const bool d1 = s.z == 1 && s.y == 1 && s.x >= 1;
+  return d3 || d2 || d1;
+}
+
+bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+
+// A validated job: its kernel variant and the record fields that do not depend on the step.
+struct JobInfo {
+  umpa::FrVariant v;
+  umpa::FrJob rec;
+  const float *table;
+  int nTable;
+};
+
+// Checks one job like umpa_fr_init + umpa_fr_execute would and derives its kernel variant.
+int check_job(const umpa_fr_job &j, int strategy, JobInfo &out)
+{
+  const umpa_fr_desc &d = j.desc;
+  const umpa_fr_settings &s = j.settings;
+  if (!size_valid(d.volume) || !size_valid(d.fft)) return UMPA_ERR_INVALID_ARG;
+  if (d.volume.x % 2 == 0 || d.volume.y != d.volume.z || d.volume.n != 1)
+    return UMPA_ERR_INVALID_ARG;
+  if (s.interpolation < 0 || s.interpolation > 1 || s.type < 0 || s.type > 1 || s.order < 0 ||
+      s.order > 4 || !(s.blobRadius > 0.f) || !std::isfinite(s.alpha))
+    return UMPA_ERR_INVALID_ARG;
+  // the work item packs z in 16 bits
+  if (d.volume.x > 65535 || d.volume.y > 65535 || d.fft.x > (uint64_t)INT_MAX ||
+      d.fft.y > (uint64_t)INT_MAX || d.fft.n > (uint64_t)INT_MAX ||
+      d.traverseSpaces > (uint64_t)INT_MAX || d.blobTable > (uint64_t)INT_MAX)
+    return UMPA_ERR_UNSUPPORTED;
+  const bool half = j.scales != nullptr;
+  if (!j.fft || !j.volume || !j.weights) return UMPA_ERR_REJECTED;
+  if (d.traverseSpaces > 0 && !j.spaces) return UMPA_ERR_INVALID_ARG;
+  if (s.interpolation == UMPA_FR_LOOKUP && (!j.table || d.blobTable == 0)) return UMPA_ERR_REJECTED;
+  if (!aligned(j.volume, 8) || !aligned(j.weights, 4) || !aligned(j.fft, half ? 4 : 8) ||
+      !aligned(j.spaces, 4) || (j.table && !aligned(j.table, 4)) ||
+      (j.scales && !aligned(j.scales, 4)))
+    return UMPA_ERR_INVALID_ARG;
+  if (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH) return UMPA_ERR_INVALID_ARG;
+
+  umpa::FrVariant v{};
+  const bool nearest = s.type == UMPA_FR_FAST;
+  if (nearest) {
+    v.mode = umpa::kModeNearest;
+  } else if (s.interpolation == UMPA_FR_LOOKUP) {
+    v.mode = umpa::kModeLookup;
+  } else {
+    v.mode = s.alpha <= 15.f ? umpa::kModeFastKaiser : umpa::kModePreciseKaiser;
+  }
+  if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
+  if (v.mode == umpa::kModeLookup && d.blobTable > (uint64_t)umpa::kFrMaxLdsTable)
+    return UMPA_ERR_UNSUPPORTED;
+  v.order = s.order;
+  const int nb = (int)std::floor(2.0 * (double)s.blobRadius + 1e-3) + 1;
+  v.nb = nb < 3 ? 3 : (nb > 6 ? 6 : nb);
+  v.half = half;
+  v.fastMath = half || strategy == UMPA_FR_FAST_MATH;
+  v.count = false;
+  v.batched = true;
+
+  umpa::FrJob a{};
+  a.volume = j.volume;
+  a.weights = j.weights;
+  a.fft = j.fft;
+  a.scales = j.scales;
+  a.spaces = j.spaces;
+  a.xSize = (int)d.fft.x;
+  a.ySize = (int)d.fft.y;
+  a.sliceElems = (int64_t)d.fft.x * (int64_t)d.fft.y;
+  a.nProj = (int)d.fft.n;
+  a.nSpaces = (int)d.traverseSpaces;
+  a.vx = (int)d.volume.x;
+  a.vy = (int)d.volume.y;
+  a.vz = (int)d.volume.z;
+  a.c = umpa::host::create_constants(d.volume.x, d.volume.y, d.blobTable, s);
+  a.cullHalfWidth = nearest ? 1.6f : s.blobRadius;
+  a.cullHalfSpace = nearest ? 0.f : s.blobRadius;
+  out.v = v;
+  out.rec = a;
+  out.table = v.mode == umpa::kModeLookup ? j.table : nullptr;
+  out.nTable = v.mode == umpa::kModeLookup ? (int)d.blobTable : 0;
+  return UMPA_OK;
+}
+
+// launches of one variant share a kernel and a blob table
+using GroupKey = std::tuple<int, int, int, bool, bool, const float *, int>;
+GroupKey group_key(const JobInfo &j)
+{
+  return GroupKey{ j.v.mode, j.v.order, j.v.nb, j.v.half, j.v.fastMath, j.table, j.nTable };
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Host image of one launch: its job records and work items, serialised for one upload.
+struct LaunchImage {
+  umpa::FrVariant v;
+  const float *table;
+  int nTable;
+  size_t jobsOff, itemsOff, counterOff;
+  int nItems;
+};
+
+}  // namespace
+
+struct umpa_fr_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool ownStream = false;
+  int numCUs = 256;
+  bool count = false;
+  unsigned long long lastUpdates = 0;
+  unsigned long long *dUpdates = nullptr;
+  std::map<std::tuple<int, int, int, float>, std::vector<umpa::host::BrickCand>> cands;
+  umpa::host::SchedParams params[2];  // schedule knobs for exact / fast-math launches
+
+  // static plan
+  std::vector<std::vector<LaunchImage>> steps;
+  char *dPlan = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  // streaming steps: pinned staging slots
+  struct Slot {
+    char *host = nullptr;
+    char *dev = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+  };
+  Slot ring[4];
+  int next = 0;
+
+  std::deque<std::pair<uint64_t, hipEvent_t>> tickets;
+  uint64_t nextTicket = 1;
+
+  const std::vector<umpa::host::BrickCand> &candidates(int vx, int vy, int vz, float depth)
+  {
+    auto key = std::make_tuple(vx, vy, vz, depth);
+    auto it = cands.find(key);
+    if (it == cands.end())
+      it = cands.emplace(key, umpa::host::brick_candidates(vx, vy, vz, depth)).first;
+    return it->second;
+  }
+
+  void drop_plan()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (dPlan) (void)hipFree(dPlan);
+    exec = nullptr;
+    graph = nullptr;
+    dPlan = nullptr;
+    steps.clear();
+  }
+};
+
+namespace {
+
+// Builds the launches of one step: `ranges` = (job index into `infos`, begin, end).  Appends
+// job records, work items and counters to `blob` (offsets relative to its start).
+void build_step(umpa_fr_batch *b, const std::vector<JobInfo> &infos,
+                const std::vector<std::tuple<int, int, int>> &ranges, std::vector<char> &blob,
+                std::vector<LaunchImage> &out, const umpa::host::SchedParams *force = nullptr)
+{
+  std::map<GroupKey, std::vector<std::tuple<int, int, int>>> groups;
+  for (const auto &r : ranges)
+    if (std::get<2>(r) > std::get<1>(r)) groups[group_key(infos[std::get<0>(r)])].push_back(r);
+  for (auto &g : groups) {
+    const auto &members = g.second;
+    const JobInfo &first = infos[std::get<0>(members[0])];
+    const umpa::host::SchedParams &sp = force ? *force : b->params[first.v.fastMath ? 1 : 0];
+    std::vector<umpa::FrJob> recs;
+    std::vector<umpa::host::JobWork> work;
+    for (size_t k = 0; k < members.size(); ++k) {
+      const auto &[ji, s0, s1] = members[k];
+      umpa::FrJob rec = infos[ji].rec;
+      rec.spaceBegin = s0;
+      rec.spaceEnd = s1;
+      recs.push_back(rec);
+      work.push_back({ &b->candidates(rec.vx, rec.vy, rec.vz, sp.depthScale), (int)k,
+                       (double)(s1 - s0) });
+    }
+    const auto items = umpa::host::make_schedule(work, first.v.fastMath, sp);
+    LaunchImage li{};
+    li.v = first.v;
+    li.table = first.table;
+    li.nTable = first.nTable;
+    li.nItems = (int)items.size();
+    li.jobsOff = align_up(blob.size(), 256);
+    blob.resize(li.jobsOff + recs.size() * sizeof(umpa::FrJob));
+    std::memcpy(blob.data() + li.jobsOff, recs.data(), recs.size() * sizeof(umpa::FrJob));
+    li.itemsOff = align_up(blob.size(), 256);
+    blob.resize(li.itemsOff + items.size() * 16);
+    int *dst = reinterpret_cast<int *>(blob.data() + li.itemsOff);
+    for (size_t i = 0; i < items.size(); ++i) {
+      dst[4 * i + 0] = items[i].x;
+      dst[4 * i + 1] = items[i].y;
+      dst[4 * i + 2] = items[i].z;
+      dst[4 * i + 3] = items[i].w;
+    }
+    li.counterOff = align_up(blob.size(), 256);
+    blob.resize(li.counterOff + 256, 0);
+    out.push_back(li);
+  }
+}
+
+// Enqueues one launch.  Kernel attributes are set beforehand (umpa_fr_batch_plan) when the
+// launch is being captured into a graph.
+hipError_t enqueue(umpa_fr_batch *b, const LaunchImage &li, char *base, bool count,
+                   bool capturing = false)
+{
+  umpa::FrLaunch l{};
+  l.jobs = reinterpret_cast<const umpa::FrJob *>(base + li.jobsOff);
+  l.items = reinterpret_cast<const int *>(base + li.itemsOff);
+  l.nItems = li.nItems;
+  l.counter = reinterpret_cast<int *>(base + li.counterOff);
+  l.updates = b->dUpdates;
+  l.table = li.table;
+  l.nTable = li.nTable;
+  umpa::FrVariant v = li.v;
+  v.count = count;
+  if (!capturing) {
+    const hipError_t e = umpa::prepare_fr_insert(v, li.nTable);
+    if (e != hipSuccess) return e;
+  }
+  umpa::FrJob unused{};
+  return umpa::launch_fr_insert(unused, l, v, b->numCUs, b->stream);
+}
+
+int validate_all(const umpa_fr_job *jobs, uint32_t nJobs, int strategy, std::vector<JobInfo> &infos)
+{
+  if (!jobs || nJobs == 0 || nJobs > 32767) return UMPA_ERR_INVALID_ARG;
+  infos.resize(nJobs);
+  for (uint32_t i = 0; i < nJobs; ++i) {
+    const int st = check_job(jobs[i], strategy, infos[i]);
+    if (st != UMPA_OK) return st;
+  }
+  return UMPA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int umpa_fr_batch_create(umpa_fr_batch **out, int device, void *stream)
+{
+  if (!out) return UMPA_ERR_INVALID_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return UMPA_ERR_DEVICE;
+  DeviceGuard g(device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  auto *b = new (std::nothrow) umpa_fr_batch();
+  if (!b) return UMPA_ERR_OUT_OF_MEMORY;
+  b->device = device;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      cus > 0)
+    b->numCUs = cus;
+  hipError_t e = hipSuccess;
+  if (stream) {
+    b->stream = static_cast<hipStream_t>(stream);
+  } else {
+    e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    b->ownStream = e == hipSuccess;
+  }
+  if (e == hipSuccess) e = hipMalloc(&b->dUpdates, sizeof(unsigned long long));
+  for (auto &s : b->ring)
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    umpa_fr_batch_destroy(b);
+    return from_hip(e);
+  }
+  *out = b;
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_plan(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                       uint32_t chunkSpaces, int strategy, int useGraph)
+{
+  if (!b || chunkSpaces == 0) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  // the previous plan's launches may still be running
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) return from_hip(e);
+  b->drop_plan();
+  std::vector<JobInfo> infos;
+  int st = validate_all(jobs, nJobs, strategy, infos);
+  if (st != UMPA_OK) return st;
+  try {
+    uint64_t maxSpaces = 0;
+    for (const auto &j : infos) maxSpaces = std::max<uint64_t>(maxSpaces, (uint64_t)j.rec.nSpaces);
+    const uint64_t nSteps = (maxSpaces + chunkSpaces - 1) / chunkSpaces;
+    std::vector<char> blob;
+    for (uint64_t c = 0; c < nSteps; ++c) {
+      std::vector<std::tuple<int, int, int>> ranges;
+      for (uint32_t i = 0; i < nJobs; ++i) {
+        const uint64_t s0 = std::min<uint64_t>(infos[i].rec.nSpaces, c * chunkSpaces);
+        const uint64_t s1 = std::min<uint64_t>(infos[i].rec.nSpaces, s0 + chunkSpaces);
+        if (s1 > s0) ranges.emplace_back((int)i, (int)s0, (int)s1);
+      }
+      std::vector<LaunchImage> launches;
+      build_step(b, infos, ranges, blob, launches);
+      b->steps.push_back(std::move(launches));
+    }
+    e = hipMalloc(&b->dPlan, std::max<size_t>(blob.size(), 256));
+    if (e == hipSuccess) e = hipMemcpy(b->dPlan, blob.data(), blob.size(), hipMemcpyHostToDevice);
+    for (const auto &step : b->steps)
+      for (const auto &li : step)
+        if (e == hipSuccess) e = umpa::prepare_fr_insert(li.v, li.nTable);
+    if (e == hipSuccess && useGraph) {
+      // the chunk loop as one graph: a counter-reset kernel and the persistent launch per
+      // (step, group), in stream order
+      e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+      hipError_t le = hipSuccess;
+      if (e == hipSuccess) {
+        for (const auto &step : b->steps)
+          for (const auto &li : step)
+            if (le == hipSuccess) le = enqueue(b, li, b->dPlan, false, true);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(b->stream, &graph);
+        b->graph = graph;
+        if (e == hipSuccess) e = le;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    }
+    if (e != hipSuccess) {
+      b->drop_plan();
+      return from_hip(e);
+    }
+    return UMPA_OK;
+  } catch (const std::bad_alloc &) {
+    b->drop_plan();
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+}
+
+int umpa_fr_batch_execute(umpa_fr_batch *b)
+{
+  if (!b) return UMPA_ERR_INVALID_ARG;
+  if (!b->dPlan) return UMPA_ERR_NOT_INITIALIZED;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  hipError_t e = hipSuccess;
+  b->lastUpdates = 0;
+  if (b->count) {
+    e = hipMemsetAsync(b->dUpdates, 0, sizeof(unsigned long long), b->stream);
+    for (const auto &step : b->steps)
+      for (const auto &li : step)
+        if (e == hipSuccess) e = enqueue(b, li, b->dPlan, true);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(&b->lastUpdates, b->dUpdates, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return from_hip(e);
+  }
+  if (b->exec) return from_hip(hipGraphLaunch(b->exec, b->stream));
+  for (const auto &step : b->steps)
+    for (const auto &li : step)
+      if (e == hipSuccess) e = enqueue(b, li, b->dPlan, false);
+  return from_hip(e);
+}
+
+int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t *begin,
+                       const uint64_t *end, uint32_t nJobs, int strategy)
+{
+  if (!b || !begin || !end) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  std::vector<JobInfo> infos;
+  int st = validate_all(jobs, nJobs, strategy, infos);
+  if (st != UMPA_OK) return st;
+  std::vector<std::tuple<int, int, int>> ranges;
+  for (uint32_t i = 0; i < nJobs; ++i) {
+    if (begin[i] > end[i] || end[i] > (uint64_t)infos[i].rec.nSpaces) return UMPA_ERR_INVALID_ARG;
+    if (end[i] > begin[i]) ranges.emplace_back((int)i, (int)begin[i], (int)end[i]);
+  }
+  b->lastUpdates = 0;
+  if (ranges.empty()) return UMPA_OK;
+  try {
+    std::vector<char> blob;
+    std::vector<LaunchImage> launches;
+    build_step(b, infos, ranges, blob, launches);
+    umpa_fr_batch::Slot &slot = b->ring[b->next];
+    b->next = (b->next + 1) % 4;
+    // the slot's previous upload and launches must be done before it is overwritten
+    hipError_t e = slot.pending ? hipEventSynchronize(slot.done) : hipSuccess;
+    slot.pending = false;
+    if (e == hipSuccess && slot.cap < blob.size()) {
+      if (slot.host) (void)hipHostFree(slot.host);
+      if (slot.dev) (void)hipFree(slot.dev);
+      slot.host = nullptr;
+      slot.dev = nullptr;
+      slot.cap = 0;
+      const size_t cap = align_up(blob.size() + blob.size() / 2, 1 << 16);
+      e = hipHostMalloc((void **)&slot.host, cap, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipMalloc(&slot.dev, cap);
+      if (e == hipSuccess) slot.cap = cap;
+    }
+    if (e == hipSuccess) {
+      std::memcpy(slot.host, blob.data(), blob.size());
+      e = hipMemcpyAsync(slot.dev, slot.host, blob.size(), hipMemcpyHostToDevice, b->stream);
+    }
+    if (e == hipSuccess && b->count)
+      e = hipMemsetAsync(b->dUpdates, 0, sizeof(unsigned long long), b->stream);
+    for (const auto &li : launches)
+      if (e == hipSuccess) e = enqueue(b, li, slot.dev, b->count);
+    if (e == hipSuccess) e = hipEventRecord(slot.done, b->stream);
+    if (e == hipSuccess) slot.pending = true;
+    if (e == hipSuccess && b->count) {
+      e = hipMemcpyAsync(&b->lastUpdates, b->dUpdates, sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    }
+    return from_hip(e);
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+}
+
+int umpa_fr_batch_num_steps(umpa_fr_batch *b, uint32_t *steps)
+{
+  if (!b || !steps) return UMPA_ERR_INVALID_ARG;
+  *steps = (uint32_t)b->steps.size();
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_set_count_updates(umpa_fr_batch *b, int enable)
+{
+  if (!b) return UMPA_ERR_INVALID_ARG;
+  b->count = enable != 0;
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_last_updates(umpa_fr_batch *b, uint64_t *updates)
+{
+  if (!b || !updates) return UMPA_ERR_INVALID_ARG;
+  *updates = b->lastUpdates;
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_record(umpa_fr_batch *b, uint64_t *ticket)
+{
+  if (!b || !ticket) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  hipEvent_t ev = nullptr;
+  hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventRecord(ev, b->stream);
+  if (e != hipSuccess) {
+    if (ev) (void)hipEventDestroy(ev);
+    return from_hip(e);
+  }
+  *ticket = b->nextTicket++;
+  b->tickets.emplace_back(*ticket, ev);
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_query(umpa_fr_batch *b, uint64_t ticket, int *done)
+{
+  if (!b || !done) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  // tickets complete in stream order: retire every finished one from the front
+  while (!b->tickets.empty()) {
+    auto &[t, ev] = b->tickets.front();
+    const hipError_t e = hipEventQuery(ev);
+    if (e == hipErrorNotReady) break;
+    (void)hipEventDestroy(ev);
+    b->tickets.pop_front();
+    if (e != hipSuccess) return from_hip(e);
+  }
+  *done = (b->tickets.empty() || b->tickets.front().first > ticket) ? 1 : 0;
+  if (ticket >= b->nextTicket) return UMPA_ERR_INVALID_ARG;
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_synchronize(umpa_fr_batch *b)
+{
+  if (!b) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  return from_hip(hipStreamSynchronize(b->stream));
+}
+
+void umpa_fr_batch_destroy(umpa_fr_batch *b)
+{
+  if (!b) return;
+  DeviceGuard g(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  b->drop_plan();
+  for (auto &s : b->ring) {
+    if (s.host) (void)hipHostFree(s.host);
+    if (s.dev) (void)hipFree(s.dev);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  for (auto &t : b->tickets) (void)hipEventDestroy(t.second);
+  if (b->dUpdates) (void)hipFree(b->dUpdates);
+  if (b->ownStream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // extern "C"

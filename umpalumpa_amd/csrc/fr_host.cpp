@@ -8,7 +8,13 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
 #include <limits>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <sys/stat.h>
 
 namespace umpa::host {
 
@@ -217,23 +223,19 @@ void compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY, const float tr
   space.weight = weight;
 }
 
-std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy)
+std::vector<BrickCand> brick_candidates(int vx, int vy, int vz, float depthScale)
 {
   // Bricks are 8 x 8 x D columns, D in {8, 4, 2, 1}.  A voxel at distance R from the centre
   // is crossed by a fraction ~ r/R of random slices, so the work of a brick grows towards the
   // centre; D shrinks with R to even the work per brick out.
-  struct Cand {
-    int x, y, z, d;
-    float est;
-  };
-  std::vector<Cand> cand;
+  std::vector<BrickCand> cand;
   const float cx = static_cast<float>((vx - 1) / 2), cy = static_cast<float>((vy - 1) / 2);
   auto radius = [&](int x, int y, int z, int d) {
     const float dx = x + 3.5f - cx, dy = y + 3.5f - cy, dz = z + 0.5f * (d - 1) - cy;
     return std::sqrt(dx * dx + dy * dy + dz * dz);
   };
-  auto depth_for = [](float R) { return R > 48.f ? 8 : R > 24.f ? 4 : R > 12.f ? 2 : 1; };
-  double total = 0;
+  const float r8 = 48.f * depthScale, r4 = 24.f * depthScale, r2 = 12.f * depthScale;
+  auto depth_for = [&](float R) { return R > r8 ? 8 : R > r4 ? 4 : R > r2 ? 2 : 1; };
   for (int y = 0; y < vy; y += 8)
     for (int x = 0; x < vx; x += 8)
       for (int z = 0; z < vz;) {
@@ -249,30 +251,121 @@ std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitH
         const float passes = std::max(1.f, 0.3f * (float)d);  // ~30% of an active brick is in the slab
         const float est = p * (68.f + 8.f * d + 300.f * passes) + 2.f;
         cand.push_back({ x, y, z, d, est });
-        total += est;
         z += d;
       }
+  return cand;
+}
+
+std::vector<BrickOrigin> make_schedule(const std::vector<JobWork> &jobs, bool splitHeavy,
+                                       const SchedParams &sp)
+{
+  double total = 0;
+  size_t count = 0;
+  for (const JobWork &j : jobs) {
+    double sum = 0;
+    for (const BrickCand &c : *j.cands) sum += c.est;
+    total += sum * j.scale;
+    count += j.cands->size();
+  }
   // a wavefront's share of the work on a full MI355X (16 wavefronts on each of 256 CUs); only
   // bricks well above both that share and the typical brick are prioritised or split
   const double fair = total / 4096.0;
-  const double mean = total / (double)std::max<size_t>(cand.size(), 1);
+  const double mean = total / (double)std::max<size_t>(count, 1);
   std::vector<BrickOrigin> bricks;
-  for (const Cand &c : cand) {
-    int flags = c.d;
-    if (c.est > fair / 4 && c.est > 4 * mean) flags |= 1 << 4;  // heavy: raised wave priority
-    int parts = 1;
-    if (splitHeavy && c.est > fair / 2 && c.est > 4 * mean)
-      parts = std::min(16, (int)std::ceil(c.est / (fair / 2)));
-    for (int j = 0; j < parts; ++j) {
-      BrickOrigin b{ c.x, c.y, c.z, flags | (parts << 8) | (j << 16), 0.f };
-      b.est = c.est / parts;
-      bricks.push_back(b);
+  bricks.reserve(count + count / 8);
+  for (const JobWork &j : jobs) {
+    for (const BrickCand &c : *j.cands) {
+      const double est = c.est * j.scale;
+      int flags = c.d;
+      if (sp.priority && est > fair / 4 && est > 4 * mean) flags |= 1 << 4;  // raised priority
+      int parts = 1;
+      const double at = fair * (double)sp.splitAt;
+      if (splitHeavy && sp.maxParts > 1 && est > at && est > 4 * mean)
+        parts = std::min(sp.maxParts, (int)std::ceil(est / at));
+      for (int k = 0; k < parts; ++k) {
+        BrickOrigin b{ c.x, c.y, c.z | (j.job << 16), flags | (parts << 8) | (k << 16), 0.f };
+        b.est = (float)(est / parts);
+        bricks.push_back(b);
+      }
     }
   }
   // heaviest first, so the long central bricks start at once
   std::stable_sort(bricks.begin(), bricks.end(),
     [](const BrickOrigin &a, const BrickOrigin &b) { return a.est > b.est; });
   return bricks;
+}
+
+std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy,
+                                             const SchedParams &sp)
+{
+  const std::vector<BrickCand> cands = brick_candidates(vx, vy, vz, sp.depthScale);
+  return make_schedule({ JobWork{ &cands, 0, 1.0 } }, splitHeavy, sp);
+}
+
+std::vector<SchedParams> tuning_candidates(bool fastMath)
+{
+  std::vector<SchedParams> out;
+  for (float depth : { 0.5f, 1.f, 2.f })
+    for (bool prio : { true, false }) {
+      if (!fastMath) {
+        out.push_back({ depth, prio, 1, 0.5f });
+        continue;
+      }
+      for (auto [parts, at] : { std::pair<int, float>{ 1, 0.5f }, { 4, 0.5f }, { 16, 0.5f },
+                                { 16, 0.25f } })
+        out.push_back({ depth, prio, parts, at });
+    }
+  return out;
+}
+
+namespace {
+std::mutex g_cacheMutex;
+
+std::string tuning_file()
+{
+  const char *dir = std::getenv("UMPA_TUNING_DIR");
+  std::string d;
+  if (dir && *dir) {
+    d = dir;
+  } else {
+    const char *home = std::getenv("HOME");
+    d = std::string(home && *home ? home : "/tmp") + "/.cache";
+    ::mkdir(d.c_str(), 0755);
+    d += "/umpalumpa_amd";
+  }
+  ::mkdir(d.c_str(), 0755);
+  return d + "/fr_tuning.txt";
+}
+}  // namespace
+
+bool tuning_cache_lookup(const std::string &key, SchedParams &out)
+{
+  std::lock_guard<std::mutex> lock(g_cacheMutex);
+  std::ifstream f(tuning_file());
+  std::string line;
+  bool found = false;
+  // the last line of a key wins
+  while (std::getline(f, line)) {
+    std::istringstream is(line);
+    std::string k;
+    SchedParams p;
+    int prio = 1;
+    double ms = 0;
+    if (is >> k >> p.depthScale >> prio >> p.maxParts >> p.splitAt >> ms && k == key) {
+      p.priority = prio != 0;
+      out = p;
+      found = true;
+    }
+  }
+  return found;
+}
+
+void tuning_cache_store(const std::string &key, const SchedParams &p, double ms)
+{
+  std::lock_guard<std::mutex> lock(g_cacheMutex);
+  std::ofstream f(tuning_file(), std::ios::app);
+  if (f) f << key << ' ' << p.depthScale << ' ' << (p.priority ? 1 : 0) << ' ' << p.maxParts
+           << ' ' << p.splitAt << ' ' << ms << '\n';
 }
 
 }  // namespace umpa::host

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "umpalumpa_amd.h"
@@ -22,16 +23,51 @@ void compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY, const float tr
                             uint32_t maxVolumeIndexYZ, bool useFast, float blobRadius,
                             float weight);
 
-// One work item of the insertion kernel: the origin of an 8 x 8 x D brick and
-// w = D | heavy << 4 | parts << 8 | part << 16 (a brick split into `parts` slice ranges is
-// accumulated by `parts` wavefronts, which merge their sums with atomics).
+// One work item of the insertion kernel: the origin of an 8 x 8 x D brick of job `job`
+// (z | job << 16) and w = D | heavy << 4 | parts << 8 | part << 16 (a brick split into `parts`
+// slice ranges is accumulated by `parts` wavefronts, which merge their sums with atomics).
 struct BrickOrigin {
   int x, y, z, w;
   float est;  // estimated cost, for ordering only
 };
 
-// Work items covering a vx*vy*vz volume, heaviest first.  `splitHeavy` splits the central
-// bricks' slice ranges (changes the summation order: fast-math strategies only).
-std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy);
+// Tunable knobs of the work-item schedule (the autotuner's search space).
+struct SchedParams {
+  float depthScale = 1.f;  // brick depth 8 / 4 / 2 / 1 beyond radius 48 / 24 / 12 times this
+  bool priority = true;    // run bricks above a quarter of a wave's fair share at priority 2
+  int maxParts = 16;       // split heavy bricks into at most this many slice ranges (fast math)
+  float splitAt = 0.5f;    // ... when their cost exceeds splitAt times a wave's fair share
+};
+
+// The bricks covering a vx*vy*vz volume with their estimated cost per inserted slice.
+struct BrickCand {
+  int x, y, z, d;
+  float est;
+};
+std::vector<BrickCand> brick_candidates(int vx, int vy, int vz, float depthScale = 1.f);
+
+// The bricks of one job in a launch, `scale` = slices the launch inserts for it.
+struct JobWork {
+  const std::vector<BrickCand> *cands;
+  int job;
+  double scale;
+};
+
+// Work items of one launch over several jobs, heaviest first.  `splitHeavy` splits the
+// heaviest bricks' slice ranges (changes the summation order: fast-math strategies only).
+std::vector<BrickOrigin> make_schedule(const std::vector<JobWork> &jobs, bool splitHeavy,
+                                       const SchedParams &sp = SchedParams());
+
+// make_schedule for a single job
+std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy,
+                                             const SchedParams &sp = SchedParams());
+
+// Persisted tuning results: one "key depthScale priority maxParts splitAt ms" line per key in
+// $UMPA_TUNING_DIR/fr_tuning.txt (default $HOME/.cache/umpalumpa_amd).
+bool tuning_cache_lookup(const std::string &key, SchedParams &out);
+void tuning_cache_store(const std::string &key, const SchedParams &p, double ms);
+
+// The autotuner's candidates for a strategy (exact: no splitting).
+std::vector<SchedParams> tuning_candidates(bool fastMath);
 
 }  // namespace umpa::host

@@ -24,7 +24,7 @@
 
 namespace umpa {
 
-constexpr int kMaxLdsTable = 12000;  // floats next to the per-wave state; larger tables stay global
+constexpr int kMaxLdsTable = kFrMaxLdsTable;  // floats next to the per-wave state; larger tables stay global
 constexpr int kWaves = 16;           // wavefronts per workgroup (they share the LDS table)
 constexpr int kThreads = 64 * kWaves;
 constexpr int kEdge = 8;             // brick edge: a wave owns 8 x 8 x 8 voxels, a z column per lane
@@ -89,7 +89,7 @@ struct Contribution {
 // ascending, ProcessVoxelBlob fr_cpu_kernels.hpp:104-147).  Used for footprints wider than the
 // unrolled NB x NB box, which only rounding at the blob edge can produce.
 template <int kMode, int kOrder, bool kHalf, bool kFastMath>
-__device__ Contribution gather_generic(const FrArgs &a, __amdgpu_buffer_rsrc_t rsrc,
+__device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buffer_rsrc_t rsrc,
                                        const float *tbl, float qx, float qy, float zSqr, float dw)
 {
   const umpa_fr_constants &c = a.c;
@@ -113,7 +113,7 @@ __device__ Contribution gather_generic(const FrArgs &a, __amdgpu_buffer_rsrc_t r
       if constexpr (kMode == kModeLookup) {
         int aux = kFastMath ? (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f)
                             : (int)(d2 * c.cIDeltaSqrt + 0.5f);
-        aux = aux > a.nTable - 1 ? a.nTable - 1 : aux;
+        aux = aux > nTable - 1 ? nTable - 1 : aux;
         wb = tbl[aux];
       } else {
         wb = blob_weight<kMode, kOrder>(d2, c);
@@ -143,9 +143,10 @@ __device__ Contribution gather_generic(const FrArgs &a, __amdgpu_buffer_rsrc_t r
 // read 0 through the bounds-checked buffer.
 // Returns false when the footprint is wider than NB.
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds>
-__device__ __forceinline__ bool gather_box(const FrArgs &a, __amdgpu_buffer_rsrc_t rsrc,
-                                           const float *tbl, float qx, float qy, float zSqr,
-                                           float dw, Contribution &o)
+__device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
+                                           __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
+                                           float qx, float qy, float zSqr, float dw,
+                                           Contribution &o)
 {
   const umpa_fr_constants &c = a.c;
   const float r = c.cBlobRadius, r2 = r * r;
@@ -187,7 +188,7 @@ __device__ __forceinline__ bool gather_box(const FrArgs &a, __amdgpu_buffer_rsrc
       if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
         // +inf and every d2 past the blob edge map to the zero entry
         int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
-        aux = aux < a.nTable ? aux : a.nTable;
+        aux = aux < nTable ? aux : nTable;
         wb = tbl[aux];
         voff = rowOff;
       } else {
@@ -196,9 +197,9 @@ __device__ __forceinline__ bool gather_box(const FrArgs &a, __amdgpu_buffer_rsrc
           int aux = kFastMath ? (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f)
                               : (int)(d2 * c.cIDeltaSqrt + 0.5f);
           if constexpr (kTableLds) {
-            wb = tbl[in ? aux : a.nTable];  // entry nTable of the LDS copy is 0
+            wb = tbl[in ? aux : nTable];  // entry nTable of the LDS copy is 0
           } else {
-            aux = aux > a.nTable - 1 ? a.nTable - 1 : aux;
+            aux = aux > nTable - 1 ? nTable - 1 : aux;
             wb = in ? tbl[aux] : 0.f;
           }
         } else {
@@ -267,57 +268,78 @@ struct WaveScratch {
   uint16_t queue[kVox];
 };
 
-template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount, bool kTableLds>
-__global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
+// Copies a wave-uniform value into scalar registers, dword by dword: a job record read from
+// memory at a uniform address then lives in SGPRs like a kernel argument.
+template <typename T>
+__device__ __forceinline__ T make_uniform(const T &v)
+{
+  static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
+  int w[sizeof(T) / 4];
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = __builtin_amdgcn_readfirstlane(w[i]);
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
+template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount, bool kTableLds,
+          bool kBatched>
+__global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrJob job0, FrLaunch l)
 {
   extern __shared__ float s_table[];
   __shared__ WaveScratch s_wave[kWaves];
+  const int nTable = l.nTable;
   if constexpr (kMode == kModeLookup && kTableLds) {
-    for (int i = threadIdx.x; i < a.nTable; i += kThreads) s_table[i] = a.table[i];
-    if (threadIdx.x == 0) s_table[a.nTable] = 0.f;
+    for (int i = threadIdx.x; i < nTable; i += kThreads) s_table[i] = l.table[i];
+    if (threadIdx.x == 0) s_table[nTable] = 0.f;
     __syncthreads();
   }
-  const float *tbl = (kMode == kModeLookup && kTableLds) ? s_table : a.table;
+  const float *tbl = (kMode == kModeLookup && kTableLds) ? s_table : l.table;
   const int lane = threadIdx.x & 63;
   WaveScratch &ws = s_wave[threadIdx.x >> 6];
   const int lx = lane & (kEdge - 1), ly = lane / kEdge;
-  const umpa_fr_constants &c = a.c;
-  const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
-  const float r = c.cBlobRadius;
-  const float r2 = r * r;
   constexpr float kHalfEdge = 0.5f * (kEdge - 1);
-  // phase-1 filter on the squared slab coordinate: the reference's own test for blobs; for the
-  // nearest-neighbour traversal the visited voxel lies within 1.5 of the plane
-  const float slab2 = kMode == kModeNearest ? 2.56f : r2;
   constexpr int kPix = kHalf ? 4 : 8;
 
   for (;;) {
     int b = 0;
-    if (lane == 0) b = atomicAdd(a.counter, 1);
+    if (lane == 0) b = atomicAdd(l.counter, 1);
     b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
-    if (b >= a.nBricks) break;
-    const int4 bo = reinterpret_cast<const int4 *>(a.bricks)[b];
-    // work item: brick 8 x 8 x depth at bo.xyz; heavy bricks run at raised wave priority; a
-    // brick split into `parts` slice ranges accumulates its range from zero and merges the
-    // partial sums with atomics (fast-math schedules only)
+    if (b >= l.nItems) break;
+    const int4 bo = make_uniform(reinterpret_cast<const int4 *>(l.items)[b]);
+    // work item: brick 8 x 8 x depth at (bo.x, bo.y, bo.z & 0xffff) of job bo.z >> 16; heavy
+    // bricks run at raised wave priority; a brick split into `parts` slice ranges accumulates
+    // its range from zero and merges the partial sums with atomics (fast-math schedules only)
+    const FrJob a = kBatched ? make_uniform(l.jobs[bo.z >> 16]) : job0;
+    const int bz = bo.z & 0xffff;
+    const int s0 = a.spaceBegin, s1 = a.spaceEnd;
+    if (s0 >= s1) continue;
     const int depth = bo.w & 15;
     const bool heavy = (bo.w >> 4) & 1;
     const int parts = (bo.w >> 8) & 255, part = (bo.w >> 16) & 255;
     const bool split = parts > 1;
-    const int sBegin = split ? (int)((int64_t)a.nSpaces * part / parts) : 0;
-    const int sEnd = split ? (int)((int64_t)a.nSpaces * (part + 1) / parts) : a.nSpaces;
+    const int sBegin = split ? s0 + (int)((int64_t)(s1 - s0) * part / parts) : s0;
+    const int sEnd = split ? s0 + (int)((int64_t)(s1 - s0) * (part + 1) / parts) : s1;
+    const umpa_fr_constants &c = a.c;
+    const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
+    const float r = c.cBlobRadius;
+    const float r2 = r * r;
+    // phase-1 filter on the squared slab coordinate: the reference's own test for blobs; for
+    // the nearest-neighbour traversal the visited voxel lies within 1.5 of the plane
+    const float slab2 = kMode == kModeNearest ? 2.56f : r2;
     if (heavy) __builtin_amdgcn_s_setprio(2);
-    // the lane owns the z column (x, y, bo.z .. bo.z + depth - 1) of the brick
+    // the lane owns the z column (x, y, bz .. bz + depth - 1) of the brick
     const int x = bo.x + lx, y = bo.y + ly;
     const bool colIn = x < a.vx && y < a.vy;
-    const int nz = min(depth, a.vz - bo.z);
+    const int nz = min(depth, a.vz - bz);
     const float px = (float)(x - cxi), py = (float)(y - cyi);
 #pragma unroll
     for (int k = 0; k < kEdge; ++k) {
       const int li = k * 64 + lane;
       float vr = 0.f, vi = 0.f, vw = 0.f;
       if (!split && colIn && k < nz) {
-        const int64_t gi = ((int64_t)(bo.z + k) * a.vy + y) * a.vx + x;
+        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
         const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
         vr = t.x;
         vi = t.y;
@@ -333,11 +355,10 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
     const float hz = 0.5f * (float)(depth - 1);
     const float bcx = (float)(bo.x - cxi) + kHalfEdge;
     const float bcy = (float)(bo.y - cyi) + kHalfEdge;
-    const float bcz = (float)(bo.z - cyi) + hz;
+    const float bcz = (float)(bz - cyi) + hz;
     const float bnorm = sqrtf(bcx * bcx + bcy * bcy + bcz * bcz);
     const float dmin = fmaxf(bnorm - sqrtf(2.f * kHalfEdge * kHalfEdge + hz * hz), 0.f);
     const float eps = 0.02f + 1e-5f * bnorm;
-
     for (int base = sBegin; base < sEnd; base += 64) {
       const int si = base + lane;
       bool pass = false;
@@ -374,7 +395,7 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
         int cnt = 0;
 #pragma unroll
         for (int k = 0; k < kEdge; ++k) {
-          const float pz = (float)(bo.z + k - cyi);
+          const float pz = (float)(bz + k - cyi);
           const float qz = cz + t22 * pz;
           const bool ok = colIn && k < nz && !(qz * qz > slab2);
           m[k] = __ballot(ok);
@@ -394,7 +415,7 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
           const bool act = item < cnt;
           const int v = act ? (int)ws.queue[item] : 0;
           const int vl = v & 63, vk = v >> 6;
-          const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bo.z + vk;
+          const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bz + vk;
           const float pxv = (float)(xv - cxi), pyv = (float)(yv - cyi), pzv = (float)(zv - cyi);
           const float dsq = pxv * pxv + pyv * pyv + pzv * pzv;
           float qx, qy, zs;
@@ -430,13 +451,13 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
               bool fits;
               if (unit) {
                 fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds>(
-                  a, rsrc, tbl, qx, qy, zs, sp.weight, o);
+                  a, nTable, rsrc, tbl, qx, qy, zs, sp.weight, o);
               } else {
                 fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds>(
-                  a, rsrc, tbl, qx, qy, zs, sp.weight, o);
+                  a, nTable, rsrc, tbl, qx, qy, zs, sp.weight, o);
               }
               if (!fits)
-                o = gather_generic<kMode, kOrder, kHalf, kFastMath>(a, rsrc, tbl, qx, qy, zs, sp.weight);
+                o = gather_generic<kMode, kOrder, kHalf, kFastMath>(a, nTable, rsrc, tbl, qx, qy, zs, sp.weight);
               if constexpr (kHalf) {
                 o.re *= scale;
                 o.im *= scale;
@@ -456,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
     for (int k = 0; k < kEdge; ++k) {
       if (colIn && k < nz) {
         const int li = k * 64 + lane;
-        const int64_t gi = ((int64_t)(bo.z + k) * a.vy + y) * a.vx + x;
+        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
         if (split) {
           float *vol = reinterpret_cast<float *>(a.volume);
           if (ws.accW[li] != 0.f || ws.accR[li] != 0.f || ws.accI[li] != 0.f) {
@@ -473,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
     wave_sync();
     if (heavy) __builtin_amdgcn_s_setprio(0);
     if constexpr (kCount) {
-      if (lane == 0 && nupd) atomicAdd(a.updates, nupd);
+      if (lane == 0 && nupd) atomicAdd(l.updates, nupd);
     }
   }
 }
@@ -481,69 +502,94 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrArgs a)
 __global__ void fr_reset_counter(int *counter) { *counter = 0; }
 
 // ---- host-side dispatch ------------------------------------------------------------------
+// The kernel is instantiated per (mode, order, footprint width, FP16, fast math, counting, LDS
+// table, batched); `visit` calls `f` with a pointer to the instantiation a variant selects.
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount>
-static hipError_t launch_one(const FrArgs &a, int numCUs, hipStream_t st)
+static const void *pick_lds(bool lds, bool batched)
 {
-  const bool lds = kMode == kModeLookup && a.nTable <= kMaxLdsTable;
-  const size_t shmem = lds ? (size_t)(a.nTable + 1) * sizeof(float) : 0;
-  int blocks = (a.nBricks + kWaves - 1) / kWaves;
-  int maxBlocks = numCUs;  // the per-wave LDS state allows one workgroup per CU
-  if (blocks > maxBlocks) blocks = maxBlocks;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(fr_reset_counter, dim3(1), dim3(1), 0, st, a.counter);
-  if (lds) {
-    auto kern = fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true>;
-    hipError_t e = hipFuncSetAttribute((const void *)kern,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(kern, dim3(blocks), dim3(kThreads), shmem, st, a);
-  } else {
-    hipLaunchKernelGGL((fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false>),
-                       dim3(blocks), dim3(kThreads), 0, st, a);
+  // batches with a lookup table keep it in LDS (tables up to kMaxLdsTable samples)
+  if constexpr (kMode == kModeLookup) {
+    if (batched && !lds) return nullptr;
+    if (batched) return (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true, true>;
   }
-  return hipGetLastError();
+  if (lds) return (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true, false>;
+  return batched ? (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false, true>
+                 : (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false, false>;
 }
 
 template <int kMode, int kOrder, bool kHalf, bool kFastMath, bool kCount>
-static hipError_t launch_nb(const FrArgs &a, int nb, int numCUs, hipStream_t st)
+static const void *pick_nb(int nb, bool lds, bool batched)
 {
   switch (nb) {
-  case 3: return launch_one<kMode, kOrder, 3, kHalf, kFastMath, kCount>(a, numCUs, st);
-  case 4: return launch_one<kMode, kOrder, 4, kHalf, kFastMath, kCount>(a, numCUs, st);
-  case 5: return launch_one<kMode, kOrder, 5, kHalf, kFastMath, kCount>(a, numCUs, st);
-  default: return launch_one<kMode, kOrder, 6, kHalf, kFastMath, kCount>(a, numCUs, st);
+  case 3: return pick_lds<kMode, kOrder, 3, kHalf, kFastMath, kCount>(lds, batched);
+  case 4: return pick_lds<kMode, kOrder, 4, kHalf, kFastMath, kCount>(lds, batched);
+  case 5: return pick_lds<kMode, kOrder, 5, kHalf, kFastMath, kCount>(lds, batched);
+  default: return pick_lds<kMode, kOrder, 6, kHalf, kFastMath, kCount>(lds, batched);
   }
 }
 
-hipError_t launch_fr_insert(const FrArgs &a, const FrVariant &v, int numCUs, hipStream_t st)
+static bool table_in_lds(const FrVariant &v, int nTable)
 {
+  return v.mode == kModeLookup && nTable <= kMaxLdsTable;
+}
+
+static const void *pick_kernel(const FrVariant &v, int nTable)
+{
+  const bool lds = table_in_lds(v, nTable);
+  const bool b = v.batched;
   switch (v.mode) {
   case kModeLookup:
     if (v.count) {
-      if (v.half) return launch_nb<kModeLookup, 0, true, true, true>(a, v.nb, numCUs, st);
-      return v.fastMath ? launch_nb<kModeLookup, 0, false, true, true>(a, v.nb, numCUs, st)
-                        : launch_nb<kModeLookup, 0, false, false, true>(a, v.nb, numCUs, st);
+      if (v.half) return pick_nb<kModeLookup, 0, true, true, true>(v.nb, lds, b);
+      return v.fastMath ? pick_nb<kModeLookup, 0, false, true, true>(v.nb, lds, b)
+                        : pick_nb<kModeLookup, 0, false, false, true>(v.nb, lds, b);
     }
-    if (v.half) return launch_nb<kModeLookup, 0, true, true, false>(a, v.nb, numCUs, st);
-    if (v.fastMath) return launch_nb<kModeLookup, 0, false, true, false>(a, v.nb, numCUs, st);
-    return launch_nb<kModeLookup, 0, false, false, false>(a, v.nb, numCUs, st);
+    if (v.half) return pick_nb<kModeLookup, 0, true, true, false>(v.nb, lds, b);
+    if (v.fastMath) return pick_nb<kModeLookup, 0, false, true, false>(v.nb, lds, b);
+    return pick_nb<kModeLookup, 0, false, false, false>(v.nb, lds, b);
   case kModeFastKaiser:
-    return v.count ? launch_one<kModeFastKaiser, 0, 4, false, false, true>(a, numCUs, st)
-                   : launch_one<kModeFastKaiser, 0, 4, false, false, false>(a, numCUs, st);
+    return v.count ? pick_lds<kModeFastKaiser, 0, 4, false, false, true>(false, b)
+                   : pick_lds<kModeFastKaiser, 0, 4, false, false, false>(false, b);
   case kModeNearest:
-    return v.count ? launch_one<kModeNearest, 0, 4, false, false, true>(a, numCUs, st)
-                   : launch_one<kModeNearest, 0, 4, false, false, false>(a, numCUs, st);
+    return v.count ? pick_lds<kModeNearest, 0, 4, false, false, true>(false, b)
+                   : pick_lds<kModeNearest, 0, 4, false, false, false>(false, b);
   case kModePreciseKaiser:
-    if (v.count) return launch_one<kModePreciseKaiser, 0, 4, false, false, true>(a, numCUs, st);
+    if (v.count) return pick_lds<kModePreciseKaiser, 0, 4, false, false, true>(false, b);
     switch (v.order) {
-    case 0: return launch_one<kModePreciseKaiser, 0, 4, false, false, false>(a, numCUs, st);
-    case 1: return launch_one<kModePreciseKaiser, 1, 4, false, false, false>(a, numCUs, st);
-    case 2: return launch_one<kModePreciseKaiser, 2, 4, false, false, false>(a, numCUs, st);
-    case 3: return launch_one<kModePreciseKaiser, 3, 4, false, false, false>(a, numCUs, st);
-    default: return launch_one<kModePreciseKaiser, 4, 4, false, false, false>(a, numCUs, st);
+    case 0: return pick_lds<kModePreciseKaiser, 0, 4, false, false, false>(false, b);
+    case 1: return pick_lds<kModePreciseKaiser, 1, 4, false, false, false>(false, b);
+    case 2: return pick_lds<kModePreciseKaiser, 2, 4, false, false, false>(false, b);
+    case 3: return pick_lds<kModePreciseKaiser, 3, 4, false, false, false>(false, b);
+    default: return pick_lds<kModePreciseKaiser, 4, 4, false, false, false>(false, b);
     }
-  default: return hipErrorInvalidValue;
+  default: return nullptr;
   }
+}
+
+hipError_t prepare_fr_insert(const FrVariant &v, int nTable)
+{
+  const void *k = pick_kernel(v, nTable);
+  if (!k) return hipErrorInvalidValue;
+  if (!table_in_lds(v, nTable)) return hipSuccess;
+  return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)((nTable + 1) * sizeof(float)));
+}
+
+hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v, int numCUs,
+                            hipStream_t st)
+{
+  const void *k = pick_kernel(v, l.nTable);
+  if (!k) return hipErrorInvalidValue;
+  const size_t shmem = table_in_lds(v, l.nTable) ? (size_t)(l.nTable + 1) * sizeof(float) : 0;
+  // persistent workgroups: the per-wave LDS state allows one per CU
+  int blocks = (l.nItems + kWaves - 1) / kWaves;
+  if (blocks > numCUs) blocks = numCUs;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(fr_reset_counter, dim3(1), dim3(1), 0, st, l.counter);
+  FrJob j = job;
+  FrLaunch ll = l;
+  void *args[] = { &j, &ll };
+  return hipLaunchKernel(k, dim3(blocks), dim3(kThreads), args, shmem, st);
 }
 
 }  // namespace umpa

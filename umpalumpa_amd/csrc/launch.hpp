@@ -9,28 +9,42 @@
 
 namespace umpa {
 
-// Argument block of one FR insertion launch.
-struct FrArgs {
+// One reconstruction as the insertion kernel sees it.  A single Execute passes it by value;
+// a batch keeps an array of them in device memory and every work item names its job.
+struct FrJob {
   void *volume;         // [z][y][x] complex float (float2)
   float *weights;       // [z][y][x]
   const void *fft;      // slices [n][ySize][xSize]: float2, or __half2 with `scales`
   const float *scales;  // per-slice dequantisation factor (FP16 slices only)
+  const umpa_traverse_space *spaces;
   int64_t sliceElems;   // xSize * ySize
   int xSize, ySize;
   int nProj;            // slices available in `fft`
   int nSpaces;
-  const umpa_traverse_space *spaces;
-  const float *table;
-  int nTable;
-  umpa_fr_constants c;
   int vx, vy, vz;       // volume extents
-  const int *bricks;    // brick origins as int4 {x, y, z, key}, heaviest first
-  int nBricks;
-  int *counter;         // work counter, zeroed by the launch itself
-  unsigned long long *updates;  // voxel-update counter (counting variant only)
+  int spaceBegin;       // this launch inserts spaces [spaceBegin, spaceEnd)
+  int spaceEnd;
+  int pad0;
+  umpa_fr_constants c;
   float cullHalfWidth;  // slab half width used by the brick cull (blob radius, 1.6 for nearest)
   float cullHalfSpace;  // allowed q.x below zero (blob radius, 0 for nearest)
 };
+static_assert(sizeof(FrJob) == 128, "job records are 128 B on host and device");
+
+// What one launch works on: work items (bricks of one or many jobs, heaviest first) and the
+// blob table shared by all its jobs.
+struct FrLaunch {
+  const FrJob *jobs;    // device array of jobs (batched launches only)
+  const int *items;     // int4 {x, y, z | job << 16, key} per work item
+  int nItems;
+  int *counter;         // work counter, zeroed before the launch
+  unsigned long long *updates;  // voxel-update counter (counting variant only)
+  const float *table;   // blob table (kLookup)
+  int nTable;
+};
+
+// Largest blob table kept in LDS (floats); batches require it there.
+constexpr int kFrMaxLdsTable = 12000;
 
 enum : int { kModeLookup = 0, kModeFastKaiser = 1, kModePreciseKaiser = 2, kModeNearest = 3 };
 
@@ -41,9 +55,15 @@ struct FrVariant {
   bool half;      // FP16 slices
   bool fastMath;  // fused multiply-add accumulation
   bool count;     // count voxel-updates
+  bool batched;   // jobs from FrLaunch::jobs (else the FrJob argument)
 };
 
-hipError_t launch_fr_insert(const FrArgs &a, const FrVariant &v, int numCUs, hipStream_t st);
+// Dynamic LDS and workgroup count of an insertion launch; `prepare` sets the kernel's LDS
+// attribute once (outside any stream capture).
+hipError_t prepare_fr_insert(const FrVariant &v, int nTable);
+// Enqueues the counter reset and the insertion kernel on `st`.
+hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v, int numCUs,
+                            hipStream_t st);
 
 struct FpArgs {
   const void *in;       // complex float

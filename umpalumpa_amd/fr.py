@@ -15,14 +15,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FR_DYNAMIC, FR_EXACT, FR_FAST, FR_FAST_MATH, FR_LOOKUP, FR_PRECISE,
-                   TRAVERSE_SPACE_DTYPE, FPDesc, FPSettings, FRConstants, FRDesc, FRSettings,
-                   Size, check)
+                   TRAVERSE_SPACE_DTYPE, FPDesc, FPSettings, FRConstants, FRDesc, FRJob,
+                   FRSettings, Size, check)
 
 __all__ = [
     "FR_DYNAMIC", "FR_LOOKUP", "FR_FAST", "FR_PRECISE", "FR_EXACT", "FR_FAST_MATH",
     "TRAVERSE_SPACE_DTYPE", "FRSettings", "FRDesc", "FPSettings", "Size",
     "FourierReconstruction", "FourierProcessing", "create_constants", "fill_blob_table",
-    "compute_traverse_space", "traverse_spaces",
+    "compute_traverse_space", "traverse_spaces", "generate_traverse_spaces", "BatchJob",
+    "FRBatch",
 ]
 
 
@@ -92,6 +93,21 @@ def traverse_spaces(rotations, padded_box: int, s: FRSettings, symmetries=None,
     return out
 
 
+def generate_traverse_spaces(rotations, padded_box: int, s: FRSettings, symmetries=None,
+                             weight: float = 1.0, first_index: int = 0) -> np.ndarray:
+    """traverse_spaces in native code (umpa_fr_generate_traverse_spaces): same records."""
+    rot = np.ascontiguousarray(np.asarray(rotations, dtype=np.float64).reshape(-1, 3, 3))
+    sym = None if symmetries is None else np.ascontiguousarray(
+        np.asarray(symmetries, dtype=np.float64).reshape(-1, 3, 3))
+    k = 1 if sym is None else len(sym)
+    out = np.zeros(len(rot) * k, dtype=TRAVERSE_SPACE_DTYPE)
+    check(_lib.load().umpa_fr_generate_traverse_spaces(
+        rot.ctypes.data_as(C.c_void_p), len(rot),
+        None if sym is None else sym.ctypes.data_as(C.c_void_p), k, padded_box, C.byref(s),
+        weight, first_index, out.ctypes.data_as(C.c_void_p)), "generate_traverse_spaces")
+    return out
+
+
 # ---- AFR ----------------------------------------------------------------------------------
 class FourierReconstruction:
     """AFR on one GPU: hand-written gfx950 voxel-owned gather kernels."""
@@ -153,6 +169,108 @@ class FourierReconstruction:
     def cleanup(self) -> None:
         check(_lib.load().umpa_fr_cleanup(self._h), "cleanup")
         self.desc = self.settings = None
+
+
+# ---- batched AFR --------------------------------------------------------------------------
+class BatchJob:
+    """One reconstruction of a batch: device tensors as for FourierReconstruction.execute."""
+
+    def __init__(self, fft, volume, weights, spaces, settings: FRSettings, table=None,
+                 scales=None, n_table: int | None = None):
+        self.fft, self.volume, self.weights, self.spaces = fft, volume, weights, spaces
+        self.settings, self.table, self.scales = settings, table, scales
+        self.n_table = (table.numel() if table is not None else 0) if n_table is None else n_table
+
+    @property
+    def n_spaces(self) -> int:
+        return self.spaces.numel() // 112
+
+    def to_c(self) -> FRJob:
+        n, hy, hx = self.fft.shape[0], self.fft.shape[1], self.fft.shape[2]
+        v = self.volume.shape[-1]
+        desc = fr_desc(hx, hy, n, v, self.n_spaces, self.n_table)
+        return FRJob(desc, self.settings, self.fft.data_ptr(),
+                     None if self.scales is None else self.scales.data_ptr(),
+                     self.volume.data_ptr(), self.weights.data_ptr(), self.spaces.data_ptr(),
+                     None if self.table is None else self.table.data_ptr())
+
+
+class FRBatch:
+    """Several reconstructions through one persistent insertion launch per step
+    (umpa_fr_batch_*).  Runs on its own stream (a HIP graph cannot capture the null stream);
+    every call first makes that stream wait for torch's current stream, so inputs written
+    there are visible, and ``wait()`` makes torch's current stream wait for the batch."""
+
+    def __init__(self, device: int = 0):
+        import torch
+        self.device = device
+        self.stream = torch.cuda.Stream(device)
+        self._h = C.c_void_p()
+        check(_lib.load().umpa_fr_batch_create(C.byref(self._h), device,
+                                               C.c_void_p(self.stream.cuda_stream)),
+              "umpa_fr_batch_create")
+        self._jobs = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().umpa_fr_batch_destroy(h)
+            self._h = None
+
+    def _follow(self):
+        import torch
+        self.stream.wait_stream(torch.cuda.current_stream(self.device))
+
+    def plan(self, jobs, chunk: int, strategy: int = FR_EXACT, graph: bool = True) -> None:
+        self._follow()
+        arr = (FRJob * len(jobs))(*[j.to_c() for j in jobs])
+        self._jobs = (jobs, arr)
+        check(_lib.load().umpa_fr_batch_plan(self._h, arr, len(jobs), chunk, strategy,
+                                             int(graph)), "umpa_fr_batch_plan")
+
+    def num_steps(self) -> int:
+        v = C.c_uint32(0)
+        check(_lib.load().umpa_fr_batch_num_steps(self._h, C.byref(v)), "num_steps")
+        return int(v.value)
+
+    def execute(self) -> None:
+        self._follow()
+        check(_lib.load().umpa_fr_batch_execute(self._h), "umpa_fr_batch_execute")
+
+    def step(self, jobs, begin, end, strategy: int = FR_EXACT) -> None:
+        self._follow()
+        n = len(jobs)
+        arr = (FRJob * n)(*[j.to_c() for j in jobs])
+        b = (C.c_uint64 * n)(*[int(x) for x in begin])
+        e = (C.c_uint64 * n)(*[int(x) for x in end])
+        check(_lib.load().umpa_fr_batch_step(self._h, arr, b, e, n, strategy),
+              "umpa_fr_batch_step")
+
+    def set_count_updates(self, enable: bool) -> None:
+        check(_lib.load().umpa_fr_batch_set_count_updates(self._h, int(enable)),
+              "set_count_updates")
+
+    def last_updates(self) -> int:
+        v = C.c_uint64(0)
+        check(_lib.load().umpa_fr_batch_last_updates(self._h, C.byref(v)), "last_updates")
+        return int(v.value)
+
+    def record(self) -> int:
+        t = C.c_uint64(0)
+        check(_lib.load().umpa_fr_batch_record(self._h, C.byref(t)), "record")
+        return int(t.value)
+
+    def query(self, ticket: int) -> bool:
+        d = C.c_int(0)
+        check(_lib.load().umpa_fr_batch_query(self._h, ticket, C.byref(d)), "query")
+        return bool(d.value)
+
+    def synchronize(self) -> None:
+        check(_lib.load().umpa_fr_batch_synchronize(self._h), "synchronize")
+
+    def wait(self) -> None:
+        import torch
+        torch.cuda.current_stream(self.device).wait_stream(self.stream)
 
 
 # ---- AFP ----------------------------------------------------------------------------------
