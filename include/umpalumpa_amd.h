@@ -201,6 +201,36 @@ int umpa_fr_batch_query(umpa_fr_batch *b, uint64_t ticket, int *done);
 int umpa_fr_batch_synchronize(umpa_fr_batch *b);
 void umpa_fr_batch_destroy(umpa_fr_batch *b);
 
+/* ---- Autotuning -------------------------------------------------------------------------
+ * The project's own tuner, in place of KTT (fr_cuda.cpp:100-157) and of its tuning data kept
+ * beside the executable (strategy_manager.cpp:32-45, utils/system.cpp:39-45).  It times every
+ * candidate schedule on scratch volumes (the caller's volumes are not touched) with the
+ * caller's slices, spaces and table, keeps the fastest for the handle, and records it in
+ * $UMPA_TUNING_DIR/fr_tuning.txt (default $HOME/.cache/umpalumpa_amd) under a key of the
+ * device, strategy, precision, footprint and problem shape; with useCache a recorded result is
+ * reused without timing.  Results are the same under every schedule for UMPA_FR_EXACT. */
+typedef struct umpa_fr_sched_params {
+  float depthScale;  /* bricks 8x8x{8,4,2,1} beyond radius {48,24,12} x depthScale */
+  int32_t priority;  /* heavy bricks at raised wave priority */
+  int32_t maxParts;  /* split heavy bricks into at most this many slice ranges (fast math) */
+  float splitAt;     /* ... above splitAt x one wavefront's fair share of the launch */
+} umpa_fr_sched_params;
+
+/* Tunes the schedule the op uses under its current strategy for inputs like these (as for
+ * umpa_fr_execute; scales != NULL for FP16 slices).  Not under graph capture (allocates). */
+int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint64_t fftCount,
+                     const umpa_traverse_space *spaces, uint64_t nSpaces, const float *table,
+                     int useCache, umpa_fr_sched_params *chosen);
+int umpa_fr_get_sched_params(umpa_fr_op *op, umpa_fr_sched_params *out);
+
+/* Tunes the schedule of a batch's later plans / steps under `strategy` on the first step of
+ * the chunk loop these jobs would form. */
+int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                           uint32_t chunkSpaces, int strategy, int useCache,
+                           umpa_fr_sched_params *chosen);
+int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy,
+                                   const umpa_fr_sched_params *p);
+
 /* ---- FR host-side helpers (no GPU needed) ----------------------------------------------- */
 
 /* AFR::CreateConstants (afr.cpp:6-19) */
@@ -227,6 +257,22 @@ int umpa_fr_generate_traverse_spaces(const double *rotations, uint64_t n,
                                      uint32_t paddedBox, const umpa_fr_settings *settings,
                                      float weight, int32_t firstIndex,
                                      umpa_traverse_space *out);
+
+/* Rotation operators of a point group, for the `symmetries` of
+ * umpa_fr_generate_traverse_spaces: "C<n>" (n rotations about z) or "D<n>" (those and n
+ * two-fold axes in the xy plane), row-major 3x3 doubles.  *count receives n or 2n; out may be
+ * NULL to query it.  The reference's harness draws random matrices instead (fr.cpp:296,
+ * fr.hpp:123-130); point groups are what a reconstruction with symmetry applies. */
+int umpa_fr_symmetry_operators(const char *group, double *out, uint32_t capacity,
+                               uint32_t *count);
+
+/* ---- Finalization (no counterpart in the reference, which stops after accumulation) ------
+ * out[k] = (V(k) + conj(V(-k))) / (W(k) + W(-k)) with `hermitian`, else V(k) / W(k): sums
+ * and division in FP64, rounded once to complex float; 0 where the weight sum is 0.  -k is
+ * the voxel mirrored through the centre (side - 1) / 2 of the side^3 volume (side odd).
+ * Enqueued on `stream` (device pointers; `out` must not alias `volume`). */
+int umpa_fr_finalize(const void *volume, const float *weights, uint64_t side, void *out,
+                     int hermitian, void *stream);
 
 /* ---- FP: the AFP operation (afp.hpp:13-53), crop/filter/normalize/center/shift ----------- */
 

@@ -71,6 +71,15 @@ class FRJob(C.Structure):
                 ("spaces", C.c_void_p), ("table", C.c_void_p)]
 
 
+class SchedParams(C.Structure):
+    """umpa_fr_sched_params: the autotuner's schedule knobs."""
+    _fields_ = [("depthScale", C.c_float), ("priority", C.c_int32), ("maxParts", C.c_int32),
+                ("splitAt", C.c_float)]
+
+    def as_tuple(self):
+        return (self.depthScale, self.priority, self.maxParts, self.splitAt)
+
+
 class FPSettings(C.Structure):
     _fields_ = [("center", C.c_int32), ("normalize", C.c_int32), ("applyFilter", C.c_int32),
                 ("shift", C.c_int32), ("hasMaxFreq", C.c_int32), ("maxFreq", C.c_float)]
@@ -106,6 +115,15 @@ PROTOTYPES = [
     ("umpa_fr_generate_traverse_spaces", C.c_int, [_P, _U64, _P, C.c_uint32, C.c_uint32,
                                                    C.POINTER(FRSettings), C.c_float, C.c_int32,
                                                    _P]),
+    ("umpa_fr_autotune", C.c_int, [_P, _P, _P, _U64, _P, _U64, _P, C.c_int,
+                                   C.POINTER(SchedParams)]),
+    ("umpa_fr_get_sched_params", C.c_int, [_P, C.POINTER(SchedParams)]),
+    ("umpa_fr_batch_autotune", C.c_int, [_P, C.POINTER(FRJob), C.c_uint32, C.c_uint32, C.c_int,
+                                         C.c_int, C.POINTER(SchedParams)]),
+    ("umpa_fr_batch_set_sched_params", C.c_int, [_P, C.c_int, C.POINTER(SchedParams)]),
+    ("umpa_fr_symmetry_operators", C.c_int, [C.c_char_p, _P, C.c_uint32,
+                                             C.POINTER(C.c_uint32)]),
+    ("umpa_fr_finalize", C.c_int, [_P, _P, _U64, _P, C.c_int, _P]),
     ("umpa_fr_batch_create", C.c_int, [C.POINTER(_P), C.c_int, _P]),
     ("umpa_fr_batch_plan", C.c_int, [_P, C.POINTER(FRJob), C.c_uint32, C.c_uint32, C.c_int,
                                      C.c_int]),

@@ -2,8 +2,11 @@
 // semantics, and dispatch to the gfx950 kernels.  Nothing here throws across the ABI.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
+#include <string>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -78,6 +81,7 @@ struct umpa_fr_op {
   int *dCounter = nullptr;
   unsigned long long *dUpdates = nullptr;
   int nb = 0;
+  umpa::host::SchedParams params[2];  // schedule knobs: exact / fast-math schedules
 
   void release()
   {
@@ -94,6 +98,22 @@ struct umpa_fr_op {
     initialized = false;
   }
 };
+
+// Work items as the kernel reads them (int4 per item) in a new device buffer.
+static hipError_t upload_bricks(const std::vector<umpa::host::BrickOrigin> &bricks, int **dst)
+{
+  std::vector<int> flat(bricks.size() * 4);
+  for (size_t i = 0; i < bricks.size(); ++i) {
+    flat[4 * i + 0] = bricks[i].x;
+    flat[4 * i + 1] = bricks[i].y;
+    flat[4 * i + 2] = bricks[i].z;
+    flat[4 * i + 3] = bricks[i].w;
+  }
+  hipError_t e = hipMalloc(dst, std::max<size_t>(flat.size(), 4) * sizeof(int));
+  if (e == hipSuccess && !flat.empty())
+    e = hipMemcpy(*dst, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
+  return e;
+}
 
 struct umpa_fp_op {
   int device = 0;
@@ -165,23 +185,10 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
     op->constants = umpa::host::create_constants(desc->volume.x, desc->volume.y,
                                                  desc->blobTable, *s);
     const int vx = (int)desc->volume.x, vy = (int)desc->volume.y, vz = (int)desc->volume.z;
-    auto upload = [](const std::vector<umpa::host::BrickOrigin> &bricks, int **dst) {
-      std::vector<int> flat(bricks.size() * 4);
-      for (size_t i = 0; i < bricks.size(); ++i) {
-        flat[4 * i + 0] = bricks[i].x;
-        flat[4 * i + 1] = bricks[i].y;
-        flat[4 * i + 2] = bricks[i].z;
-        flat[4 * i + 3] = bricks[i].w;
-      }
-      hipError_t e = hipMalloc(dst, flat.size() * sizeof(int));
-      if (e == hipSuccess)
-        e = hipMemcpy(*dst, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice);
-      return e;
-    };
-    auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, false);
-    auto split = umpa::host::make_brick_schedule(vx, vy, vz, true);
-    hipError_t e = upload(bricks, &op->dBricks);
-    if (e == hipSuccess) e = upload(split, &op->dBricksSplit);
+    auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, false, op->params[0]);
+    auto split = umpa::host::make_brick_schedule(vx, vy, vz, true, op->params[1]);
+    hipError_t e = upload_bricks(bricks, &op->dBricks);
+    if (e == hipSuccess) e = upload_bricks(split, &op->dBricksSplit);
     if (e == hipSuccess) e = hipMalloc(&op->dCounter, sizeof(int));
     if (e == hipSuccess) e = hipMalloc(&op->dUpdates, sizeof(unsigned long long));
     if (e != hipSuccess) {
@@ -205,10 +212,11 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
   }
 }
 
-static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales, bool half,
-                           uint64_t fftCount, void *volume, float *weights,
-                           const umpa_traverse_space *spaces, uint64_t nSpaces,
-                           const float *table)
+// Validation shared by execute and autotune: the reference's Execute preconditions.
+static int fr_check_execute(umpa_fr_op *op, const void *fft, const float *scales, bool half,
+                            uint64_t fftCount, const void *volume, const float *weights,
+                            const umpa_traverse_space *spaces, uint64_t nSpaces,
+                            const float *table)
 {
   if (!op) return UMPA_ERR_INVALID_ARG;
   if (!op->initialized) return UMPA_ERR_NOT_INITIALIZED;
@@ -225,11 +233,13 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
       !aligned(spaces, 4) || (table && !aligned(table, 4)))
     return UMPA_ERR_INVALID_ARG;
   if (half && !scales) return UMPA_ERR_INVALID_ARG;
-  op->lastUpdates = 0;
-  if (nSpaces == 0) return UMPA_OK;
-  DeviceGuard g(op->device);
-  if (!g.ok) return UMPA_ERR_DEVICE;
+  return UMPA_OK;
+}
 
+static umpa::FrJob fr_make_job(const umpa_fr_op *op, const void *fft, const float *scales,
+                               uint64_t fftCount, void *volume, float *weights,
+                               const umpa_traverse_space *spaces, uint64_t nSpaces)
+{
   umpa::FrJob a{};
   a.volume = volume;
   a.weights = weights;
@@ -250,18 +260,13 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
   const bool nearest = op->settings.type == UMPA_FR_FAST;
   a.cullHalfWidth = nearest ? 1.6f : op->settings.blobRadius;
   a.cullHalfSpace = nearest ? 0.f : op->settings.blobRadius;
-  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
-  const bool fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
-  umpa::FrLaunch l{};
-  l.items = fastMath ? op->dBricksSplit : op->dBricks;
-  l.nItems = fastMath ? op->nBricksSplit : op->nBricks;
-  l.counter = op->dCounter;
-  l.updates = op->dUpdates;
-  l.table = table;
-  l.nTable = (int)op->desc.blobTable;
+  return a;
+}
 
+static umpa::FrVariant fr_make_variant(const umpa_fr_op *op, bool half)
+{
   umpa::FrVariant v{};
-  if (nearest) {
+  if (op->settings.type == UMPA_FR_FAST) {
     v.mode = umpa::kModeNearest;
   } else if (op->settings.interpolation == UMPA_FR_LOOKUP) {
     v.mode = umpa::kModeLookup;
@@ -272,10 +277,36 @@ static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales,
   v.order = op->settings.order;
   v.nb = op->nb;
   v.half = half;
-  v.fastMath = fastMath;
+  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
+  v.fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
   v.count = op->countUpdates;
   v.batched = false;
+  return v;
+}
+
+static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales, bool half,
+                           uint64_t fftCount, void *volume, float *weights,
+                           const umpa_traverse_space *spaces, uint64_t nSpaces,
+                           const float *table)
+{
+  int st = fr_check_execute(op, fft, scales, half, fftCount, volume, weights, spaces, nSpaces,
+                            table);
+  if (st != UMPA_OK) return st;
+  op->lastUpdates = 0;
+  if (nSpaces == 0) return UMPA_OK;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+
+  const umpa::FrJob a = fr_make_job(op, fft, scales, fftCount, volume, weights, spaces, nSpaces);
+  const umpa::FrVariant v = fr_make_variant(op, half);
   if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
+  umpa::FrLaunch l{};
+  l.items = v.fastMath ? op->dBricksSplit : op->dBricks;
+  l.nItems = v.fastMath ? op->nBricksSplit : op->nBricks;
+  l.counter = op->dCounter;
+  l.updates = op->dUpdates;
+  l.table = table;
+  l.nTable = (int)op->desc.blobTable;
   hipError_t e = hipSuccess;
   if (v.count) e = hipMemsetAsync(op->dUpdates, 0, sizeof(unsigned long long), op->stream);
   if (e == hipSuccess) e = umpa::prepare_fr_insert(v, l.nTable);
@@ -303,6 +334,117 @@ int umpa_fr_execute_f16(umpa_fr_op *op, const void *fft, const float *scales,
 {
   return fr_execute_impl(op, fft, scales, true, fftCount, volume, weights, spaces, nSpaces,
                          table);
+}
+
+int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint64_t fftCount,
+                     const umpa_traverse_space *spaces, uint64_t nSpaces, const float *table,
+                     int useCache, umpa_fr_sched_params *chosen)
+{
+  const bool half = scales != nullptr;
+  // the checks of execute, with a placeholder for the output (scratch volumes are used)
+  alignas(8) static const float kAligned[2] = { 0.f, 0.f };
+  int st = fr_check_execute(op, fft, scales, half, fftCount, kAligned, kAligned, spaces, nSpaces,
+                            table);
+  if (st != UMPA_OK) return st;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  umpa::FrVariant v = fr_make_variant(op, half);
+  if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
+  v.count = false;
+  const int slot = v.fastMath ? 1 : 0;
+  const int vx = (int)op->desc.volume.x, vy = (int)op->desc.volume.y, vz = (int)op->desc.volume.z;
+  // spaces bucketed to a power of two
+  uint64_t bucket = 1;
+  while (bucket < nSpaces) bucket <<= 1;
+  const std::string key = "op:cu" + std::to_string(op->numCUs) + ":m" + std::to_string(v.mode) +
+                          ":f" + std::to_string(slot) + ":h" + std::to_string(half ? 1 : 0) +
+                          ":nb" + std::to_string(v.nb) + ":" + std::to_string(vx) + "x" +
+                          std::to_string(bucket);
+  umpa::host::SchedParams best = op->params[slot];
+  bool found = useCache && umpa::host::tuning_cache_lookup(key, best);
+  hipError_t e = hipSuccess;
+  double bestMs = 1e300;
+  if (!found && nSpaces > 0) {
+    const size_t n = (size_t)vx * vy * vz;
+    void *vol = nullptr, *wts = nullptr;
+    int *items = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    e = hipStreamSynchronize(op->stream);
+    if (e == hipSuccess) e = hipMalloc(&vol, n * 8);
+    if (e == hipSuccess) e = hipMalloc(&wts, n * 4);
+    if (e == hipSuccess) e = hipMemsetAsync(vol, 0, n * 8, op->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(wts, 0, n * 4, op->stream);
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) e = umpa::prepare_fr_insert(v, (int)op->desc.blobTable);
+    const umpa::FrJob a = fr_make_job(op, fft, scales, fftCount, vol, static_cast<float *>(wts),
+                                      spaces, nSpaces);
+    try {
+      for (const auto &cand : umpa::host::tuning_candidates(v.fastMath)) {
+        if (e != hipSuccess) break;
+        const auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, v.fastMath, cand);
+        if (items) (void)hipFree(items);
+        items = nullptr;
+        e = upload_bricks(bricks, &items);
+        umpa::FrLaunch l{};
+        l.items = items;
+        l.nItems = (int)bricks.size();
+        l.counter = op->dCounter;
+        l.updates = op->dUpdates;
+        l.table = table;
+        l.nTable = (int)op->desc.blobTable;
+        double ms = 1e300;
+        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+          e = hipEventRecord(t0, op->stream);
+          if (e == hipSuccess) e = umpa::launch_fr_insert(a, l, v, op->numCUs, op->stream);
+          if (e == hipSuccess) e = hipEventRecord(t1, op->stream);
+          if (e == hipSuccess) e = hipEventSynchronize(t1);
+          float t = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+          if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms up
+        }
+        if (e == hipSuccess && ms < bestMs) {
+          bestMs = ms;
+          best = cand;
+        }
+      }
+    } catch (const std::bad_alloc &) {
+      e = hipErrorOutOfMemory;
+    }
+    (void)hipStreamSynchronize(op->stream);
+    if (items) (void)hipFree(items);
+    if (vol) (void)hipFree(vol);
+    if (wts) (void)hipFree(wts);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (e != hipSuccess) return from_hip(e);
+    umpa::host::tuning_cache_store(key, best, bestMs);
+  }
+  // rebuild the schedule of this strategy with the chosen knobs
+  try {
+    const auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, v.fastMath, best);
+    int *items = nullptr;
+    e = upload_bricks(bricks, &items);
+    if (e != hipSuccess) return from_hip(e);
+    int *&dst = v.fastMath ? op->dBricksSplit : op->dBricks;
+    (void)hipFree(dst);
+    dst = items;
+    (v.fastMath ? op->nBricksSplit : op->nBricks) = (int)bricks.size();
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+  op->params[slot] = best;
+  if (chosen) *chosen = { best.depthScale, best.priority ? 1 : 0, best.maxParts, best.splitAt };
+  return UMPA_OK;
+}
+
+int umpa_fr_get_sched_params(umpa_fr_op *op, umpa_fr_sched_params *out)
+{
+  if (!op || !out) return UMPA_ERR_INVALID_ARG;
+  const int slot = op->strategy == UMPA_FR_FAST_MATH ? 1 : 0;
+  const auto &p = op->params[slot];
+  *out = { p.depthScale, p.priority ? 1 : 0, p.maxParts, p.splitAt };
+  return UMPA_OK;
 }
 
 int umpa_fr_set_strategy(umpa_fr_op *op, int strategy)
@@ -404,6 +546,31 @@ int umpa_fr_generate_traverse_spaces(const double *rotations, uint64_t n,
     }
   }
   return UMPA_OK;
+}
+
+int umpa_fr_symmetry_operators(const char *group, double *out, uint32_t capacity,
+                               uint32_t *count)
+{
+  if (!group || !count) return UMPA_ERR_INVALID_ARG;
+  const char f = group[0] == 'c' ? 'C' : group[0] == 'd' ? 'D' : group[0];
+  char *end = nullptr;
+  const long n = std::strtol(group + 1, &end, 10);
+  if ((f != 'C' && f != 'D') || !end || *end != '\0' || end == group + 1 || n < 1 || n > 1000)
+    return UMPA_ERR_INVALID_ARG;
+  return umpa::host::symmetry_operators(f, (int)n, out, capacity, count) == 0
+           ? UMPA_OK : UMPA_ERR_INVALID_ARG;
+}
+
+// ---- finalization -----------------------------------------------------------------------
+int umpa_fr_finalize(const void *volume, const float *weights, uint64_t side, void *out,
+                     int hermitian, void *stream)
+{
+  if (!volume || !weights || !out || side < 1 || side % 2 == 0 || side > 65535)
+    return UMPA_ERR_INVALID_ARG;
+  if (!aligned(volume, 8) || !aligned(weights, 4) || !aligned(out, 8))
+    return UMPA_ERR_INVALID_ARG;
+  return from_hip(umpa::launch_fr_finalize(volume, weights, (int)side, out, hermitian != 0,
+                                           static_cast<hipStream_t>(stream)));
 }
 
 // ---- FP ---------------------------------------------------------------------------------

@@ -19,6 +19,7 @@
 #include <deque>
 #include <map>
 #include <new>
+#include <string>
 #include <tuple>
 #include <utility>
 #include <vector>
@@ -476,6 +477,116 @@ int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t
   } catch (const std::bad_alloc &) {
     return UMPA_ERR_OUT_OF_MEMORY;
   }
+}
+
+int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                           uint32_t chunkSpaces, int strategy, int useCache,
+                           umpa_fr_sched_params *chosen)
+{
+  if (!b || chunkSpaces == 0) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(b->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  std::vector<JobInfo> infos;
+  int st = validate_all(jobs, nJobs, strategy, infos);
+  if (st != UMPA_OK) return st;
+  const bool fastMath = infos[0].v.fastMath;
+  for (const auto &j : infos)
+    if (j.v.fastMath != fastMath) return UMPA_ERR_INVALID_ARG;
+  // key: device, strategy, precision, footprint, and the first step's (volume, spaces) list
+  std::vector<std::pair<int, int>> shape;
+  std::vector<std::tuple<int, int, int>> ranges;
+  for (uint32_t i = 0; i < nJobs; ++i) {
+    const int s1 = std::min<int>(infos[i].rec.nSpaces, (int)chunkSpaces);
+    if (s1 > 0) ranges.emplace_back((int)i, 0, s1);
+    shape.emplace_back(infos[i].rec.vx, s1);
+  }
+  std::sort(shape.begin(), shape.end());
+  std::string key = "batch:cu" + std::to_string(b->numCUs) + ":m" +
+                    std::to_string(infos[0].v.mode) + ":f" + std::to_string(fastMath ? 1 : 0) +
+                    ":h" + std::to_string(infos[0].v.half ? 1 : 0) + ":nb" +
+                    std::to_string(infos[0].v.nb) + ":";
+  for (auto &p : shape) key += std::to_string(p.first) + "x" + std::to_string(p.second) + ",";
+  umpa::host::SchedParams best;
+  auto report = [&](const umpa::host::SchedParams &p) {
+    b->params[fastMath ? 1 : 0] = p;
+    if (chosen) *chosen = { p.depthScale, p.priority ? 1 : 0, p.maxParts, p.splitAt };
+    return UMPA_OK;
+  };
+  if (useCache && umpa::host::tuning_cache_lookup(key, best)) return report(best);
+  if (ranges.empty()) return report(b->params[fastMath ? 1 : 0]);
+
+  hipError_t e = hipStreamSynchronize(b->stream);
+  // scratch volumes: the candidates accumulate into these, never into the caller's
+  std::vector<void *> scratch;
+  for (auto &j : infos) {
+    const size_t n = (size_t)j.rec.vx * j.rec.vy * j.rec.vz;
+    void *v = nullptr, *w = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&v, n * 8);
+    if (e == hipSuccess) scratch.push_back(v);
+    if (e == hipSuccess) e = hipMalloc(&w, n * 4);
+    if (e == hipSuccess) scratch.push_back(w);
+    if (e == hipSuccess) e = hipMemsetAsync(v, 0, n * 8, b->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(w, 0, n * 4, b->stream);
+    j.rec.volume = v;
+    j.rec.weights = static_cast<float *>(w);
+  }
+  char *dBlob = nullptr;
+  size_t cap = 0;
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&t0);
+  if (e == hipSuccess) e = hipEventCreate(&t1);
+  double bestMs = 1e300;
+  try {
+    for (const auto &cand : umpa::host::tuning_candidates(fastMath)) {
+      if (e != hipSuccess) break;
+      std::vector<char> blob;
+      std::vector<LaunchImage> launches;
+      build_step(b, infos, ranges, blob, launches, &cand);
+      if (blob.size() > cap) {
+        if (dBlob) (void)hipFree(dBlob);
+        dBlob = nullptr;
+        cap = align_up(blob.size() * 2, 1 << 16);
+        e = hipMalloc(&dBlob, cap);
+        if (e != hipSuccess) break;
+      }
+      e = hipMemcpy(dBlob, blob.data(), blob.size(), hipMemcpyHostToDevice);
+      double ms = 1e300;
+      for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+        e = hipEventRecord(t0, b->stream);
+        for (const auto &li : launches)
+          if (e == hipSuccess) e = enqueue(b, li, dBlob, false);
+        if (e == hipSuccess) e = hipEventRecord(t1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+        if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms up
+      }
+      if (e == hipSuccess && ms < bestMs) {
+        bestMs = ms;
+        best = cand;
+      }
+    }
+  } catch (const std::bad_alloc &) {
+    e = hipErrorOutOfMemory;
+  }
+  (void)hipStreamSynchronize(b->stream);
+  if (dBlob) (void)hipFree(dBlob);
+  for (void *p : scratch) (void)hipFree(p);
+  if (t0) (void)hipEventDestroy(t0);
+  if (t1) (void)hipEventDestroy(t1);
+  if (e != hipSuccess) return from_hip(e);
+  umpa::host::tuning_cache_store(key, best, bestMs);
+  return report(best);
+}
+
+int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy, const umpa_fr_sched_params *p)
+{
+  if (!b || !p || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH) ||
+      !(p->depthScale > 0.f) || p->maxParts < 1 || p->maxParts > 255 || !(p->splitAt > 0.f))
+    return UMPA_ERR_INVALID_ARG;
+  b->params[strategy == UMPA_FR_FAST_MATH ? 1 : 0] =
+    umpa::host::SchedParams{ p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
+  return UMPA_OK;
 }
 
 int umpa_fr_batch_num_steps(umpa_fr_batch *b, uint32_t *steps)

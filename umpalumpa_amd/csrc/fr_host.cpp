@@ -302,6 +302,32 @@ std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitH
   return make_schedule({ JobWork{ &cands, 0, 1.0 } }, splitHeavy, sp);
 }
 
+int symmetry_operators(char family, int n, double *out, uint32_t cap, uint32_t *count)
+{
+  // C_n: rotations by 2 pi k / n about z.  D_n: C_n and the n two-fold rotations about the
+  // axes at angles pi k / n in the xy plane (z -> -z).
+  if (n < 1 || !count || (family != 'C' && family != 'D')) return -1;
+  const uint32_t total = (uint32_t)(family == 'C' ? n : 2 * n);
+  *count = total;
+  if (!out) return 0;
+  if (cap < total) return -1;
+  const double pi = 3.14159265358979323846;
+  uint32_t k = 0;
+  for (int i = 0; i < n; ++i, ++k) {
+    const double a = 2.0 * pi * i / n, c = std::cos(a), s = std::sin(a);
+    const double m[9] = { c, -s, 0, s, c, 0, 0, 0, 1 };
+    std::copy(m, m + 9, out + 9 * k);
+  }
+  if (family == 'D') {
+    for (int i = 0; i < n; ++i, ++k) {
+      const double t = pi * i / n, c = std::cos(2 * t), s = std::sin(2 * t);
+      const double m[9] = { c, s, 0, s, -c, 0, 0, 0, -1 };
+      std::copy(m, m + 9, out + 9 * k);
+    }
+  }
+  return 0;
+}
+
 std::vector<SchedParams> tuning_candidates(bool fastMath)
 {
   std::vector<SchedParams> out;

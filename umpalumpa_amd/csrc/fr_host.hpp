@@ -67,6 +67,10 @@ std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitH
 bool tuning_cache_lookup(const std::string &key, SchedParams &out);
 void tuning_cache_store(const std::string &key, const SchedParams &p, double ms);
 
+// Rotation matrices (row-major 3x3, FP64) of the point group C_n or D_n ('C' / 'D'); the
+// number is written to *count (n or 2n); out == nullptr only queries it.  0 on success.
+int symmetry_operators(char family, int n, double *out, uint32_t cap, uint32_t *count);
+
 // The autotuner's candidates for a strategy (exact: no splitting).
 std::vector<SchedParams> tuning_candidates(bool fastMath);
 

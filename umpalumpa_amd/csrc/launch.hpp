@@ -65,6 +65,11 @@ hipError_t prepare_fr_insert(const FrVariant &v, int nTable);
 hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v, int numCUs,
                             hipStream_t st);
 
+// Finalization of a (side^3) volume: optional Hermitian completion and the FP64 division by
+// the weights, into `out` (complex float).
+hipError_t launch_fr_finalize(const void *volume, const float *weights, int side, void *out,
+                              bool hermitian, hipStream_t st);
+
 struct FpArgs {
   const void *in;       // complex float
   void *out;            // complex float

@@ -16,14 +16,14 @@ import numpy as np
 from . import _lib
 from ._lib import (FR_DYNAMIC, FR_EXACT, FR_FAST, FR_FAST_MATH, FR_LOOKUP, FR_PRECISE,
                    TRAVERSE_SPACE_DTYPE, FPDesc, FPSettings, FRConstants, FRDesc, FRJob,
-                   FRSettings, Size, check)
+                   FRSettings, SchedParams, Size, check)
 
 __all__ = [
     "FR_DYNAMIC", "FR_LOOKUP", "FR_FAST", "FR_PRECISE", "FR_EXACT", "FR_FAST_MATH",
     "TRAVERSE_SPACE_DTYPE", "FRSettings", "FRDesc", "FPSettings", "Size",
     "FourierReconstruction", "FourierProcessing", "create_constants", "fill_blob_table",
     "compute_traverse_space", "traverse_spaces", "generate_traverse_spaces", "BatchJob",
-    "FRBatch",
+    "FRBatch", "SchedParams", "symmetry_operators", "finalize",
 ]
 
 
@@ -108,6 +108,31 @@ def generate_traverse_spaces(rotations, padded_box: int, s: FRSettings, symmetri
     return out
 
 
+def symmetry_operators(group: str) -> np.ndarray:
+    """Rotation operators (k, 3, 3) float64 of the point group "C<n>" or "D<n>"."""
+    lib = _lib.load()
+    n = C.c_uint32(0)
+    check(lib.umpa_fr_symmetry_operators(group.encode(), None, 0, C.byref(n)),
+          f"symmetry_operators({group})")
+    out = np.zeros((n.value, 3, 3), dtype=np.float64)
+    check(lib.umpa_fr_symmetry_operators(group.encode(), out.ctypes.data_as(C.c_void_p),
+                                         n.value, C.byref(n)), f"symmetry_operators({group})")
+    return out
+
+
+def finalize(volume, weights, out=None, hermitian: bool = True, stream=None):
+    """Hermitian completion and FP64 weight normalisation of an inserted volume
+    (umpa_fr_finalize) into a new (or the given) complex64 tensor on the same device."""
+    import torch
+    side = volume.shape[-1]
+    if out is None:
+        out = torch.empty_like(volume)
+    check(_lib.load().umpa_fr_finalize(_ptr(volume), _ptr(weights), side, _ptr(out),
+                                       int(hermitian), _stream_handle(stream)),
+          "umpa_fr_finalize")
+    return out
+
+
 # ---- AFR ----------------------------------------------------------------------------------
 class FourierReconstruction:
     """AFR on one GPU: hand-written gfx950 voxel-owned gather kernels."""
@@ -162,6 +187,22 @@ class FourierReconstruction:
                 self._h, _ptr(fft), _ptr(scales), fft.shape[0], _ptr(volume), _ptr(weights),
                 _ptr(spaces), n_sp, _ptr(table))
         return self.last_status == 0
+
+    def autotune(self, fft, spaces, table=None, n_spaces=None, scales=None,
+                 use_cache: bool = True) -> SchedParams:
+        """Times the candidate schedules for inputs like these on scratch volumes and keeps
+        the fastest for the current strategy (umpa_fr_autotune)."""
+        n_sp = spaces.numel() // 112 if n_spaces is None else n_spaces
+        out = SchedParams()
+        check(_lib.load().umpa_fr_autotune(self._h, _ptr(fft), _ptr(scales), fft.shape[0],
+                                           _ptr(spaces), n_sp, _ptr(table), int(use_cache),
+                                           C.byref(out)), "umpa_fr_autotune")
+        return out
+
+    def sched_params(self) -> SchedParams:
+        out = SchedParams()
+        check(_lib.load().umpa_fr_get_sched_params(self._h, C.byref(out)), "sched_params")
+        return out
 
     def synchronize(self) -> None:
         check(_lib.load().umpa_fr_synchronize(self._h), "synchronize")
@@ -227,6 +268,21 @@ class FRBatch:
         self._jobs = (jobs, arr)
         check(_lib.load().umpa_fr_batch_plan(self._h, arr, len(jobs), chunk, strategy,
                                              int(graph)), "umpa_fr_batch_plan")
+
+    def autotune(self, jobs, chunk: int, strategy: int = FR_EXACT,
+                 use_cache: bool = True) -> SchedParams:
+        """Tunes the schedule of later plans / steps on the first step of these jobs."""
+        self._follow()
+        arr = (FRJob * len(jobs))(*[j.to_c() for j in jobs])
+        out = SchedParams()
+        check(_lib.load().umpa_fr_batch_autotune(self._h, arr, len(jobs), chunk, strategy,
+                                                 int(use_cache), C.byref(out)),
+              "umpa_fr_batch_autotune")
+        return out
+
+    def set_sched_params(self, strategy: int, p: SchedParams) -> None:
+        check(_lib.load().umpa_fr_batch_set_sched_params(self._h, strategy, C.byref(p)),
+              "set_sched_params")
 
     def num_steps(self) -> int:
         v = C.c_uint32(0)
