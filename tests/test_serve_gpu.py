@@ -86,18 +86,22 @@ def test_f16_alone_vs_mix_within_tolerance():
         assert T.rel_l2(jm.weights.cpu().numpy(), aw) <= 1e-6
 
 
-def test_budget_admission_serialises_jobs():
-    """A budget that holds one job at a time: no two jobs overlap, all complete, and results
-    are unchanged."""
+def test_budget_admission_respects_the_budget():
+    """A budget that holds the largest job but not all jobs: at every admission the admitted
+    footprints fit the budget, all jobs complete, and results are unchanged."""
     chunk = 64
     jobs = make_jobs(chunk=chunk)
-    biggest = max(j.footprint(chunk) for j, _ in jobs)
-    srv = S.Server(chunk=chunk, strategy=F.FR_EXACT, hbm_budget=biggest * 1.01)
+    foot = {j.job_id: j.footprint(chunk) for j, _ in jobs}
+    budget = max(foot.values()) * 1.01
+    assert sum(foot.values()) > budget
+    srv = S.Server(chunk=chunk, strategy=F.FR_EXACT, hbm_budget=budget)
     done = srv.run([j for j, _ in jobs], realtime=False)
     assert len(done) == len(jobs)
-    spans = sorted((j.admitted, j.done) for j in done)
-    for (a0, d0), (a1, d1) in zip(spans, spans[1:]):
-        assert a1 >= d0 - 1e-9
+    for j in done:
+        # jobs holding memory when j was admitted (the server releases a job's budget at the
+        # moment it records the job's completion)
+        live = [k for k in done if k.admitted <= j.admitted < k.done]
+        assert sum(foot[k.job_id] for k in live) <= budget
     alone = make_jobs(chunk=chunk)
     for (jm, _), ja in zip(jobs, alone):
         av, aw = solo(ja, F.FR_EXACT, chunk)

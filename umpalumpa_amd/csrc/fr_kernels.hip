@@ -46,27 +46,77 @@ __device__ __forceinline__ unsigned lanes_below(unsigned long long mask)
 
 // Raw buffer loads through the LLVM intrinsics.  (The clang builtin for the 64-bit raw buffer
 // load, __builtin_amdgcn_raw_buffer_load_b64, lowers to a single 32-bit load in this toolchain
-// and returns that dword for both halves, so the vector form is declared directly.)
+// and returns that dword for both halves, so the vector forms are declared directly.)  Every
+// byte offset travels in the vector offset, which the hardware range-checks together with the
+// instruction's immediate offset; the scalar offset is not range-checked, and loads that
+// differed only in it were observed to be merged into one by this compiler, so it stays 0.
 typedef float f32x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 __device__ f32x2_t raw_buffer_load_f32x2(__amdgpu_buffer_rsrc_t rsrc, uint32_t voffset,
                                         uint32_t soffset,
                                         int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v2f32");
 __device__ uint32_t raw_buffer_load_u32(__amdgpu_buffer_rsrc_t rsrc, uint32_t voffset,
                                         uint32_t soffset,
                                         int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.i32");
+__device__ f32x4_t raw_buffer_load_f32x4(__amdgpu_buffer_rsrc_t rsrc, uint32_t voffset,
+                                        uint32_t soffset,
+                                        int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4f32");
+__device__ u32x4_t raw_buffer_load_u32x4(__amdgpu_buffer_rsrc_t rsrc, uint32_t voffset,
+                                        uint32_t soffset,
+                                        int aux) __asm("llvm.amdgcn.raw.ptr.buffer.load.v4i32");
 
-// Slice pixel through a bounds-checked buffer resource: offsets at or past the slice end
-// return zero instead of faulting.
+// Slice pixel at byte offset `byteOffset` through a bounds-checked buffer resource: offsets at
+// or past the slice end return zero instead of faulting.
 template <bool kHalf>
-__device__ __forceinline__ float2 load_pixel(__amdgpu_buffer_rsrc_t rsrc, uint32_t byteOffset,
-                                             uint32_t scalarOffset = 0)
+__device__ __forceinline__ float2 load_pixel(__amdgpu_buffer_rsrc_t rsrc, uint32_t byteOffset)
 {
   if constexpr (kHalf) {
-    const uint32_t v = raw_buffer_load_u32(rsrc, byteOffset, scalarOffset, 0);
+    const uint32_t v = raw_buffer_load_u32(rsrc, byteOffset, 0, 0);
     return __half22float2(__builtin_bit_cast(__half2, v));
   } else {
-    const f32x2_t v = raw_buffer_load_f32x2(rsrc, byteOffset, scalarOffset, 0);
-    return make_float2(v[0], v[1]);
+    const f32x2_t v = raw_buffer_load_f32x2(rsrc, byteOffset, 0, 0);
+    return make_float2(v.x, v.y);
+  }
+}
+
+// kNB consecutive pixels of a slice row starting at byte `voff`, with 16-byte loads (4 FP16 or
+// 2 FP32 pixels each).  A load that starts past the slice end reads zeros.  Only the slice's
+// last row can have a load that straddles the end; such a row is read pixel by pixel, so no
+// result depends on how the hardware range-checks a partially valid access.
+template <bool kHalf, int kNB>
+__device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff,
+                                         uint32_t sliceBytes, float2 (&px)[kNB])
+{
+  constexpr int kPix = kHalf ? 4 : 8;
+  constexpr int kPer = 16 / kPix;                  // pixels per 16-byte load
+  constexpr int kLoads = (kNB + kPer - 1) / kPer;
+  if (voff < sliceBytes && voff + (uint32_t)(kLoads * 16) > sliceBytes) {
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) px[j] = load_pixel<kHalf>(rsrc, voff + (uint32_t)(j * kPix));
+    return;
+  }
+  // (vector elements are named, never indexed by a loop variable: this compiler folded
+  // v[e] in an unrolled loop over e to v[0] for every e, in the 16-byte loads here)
+#pragma unroll
+  for (int q = 0; q < kLoads; ++q) {
+    if constexpr (kHalf) {
+      const u32x4_t v = raw_buffer_load_u32x4(rsrc, voff + (uint32_t)(q * 16), 0, 0);
+      const uint32_t w[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int j = q * kPer + e;
+        if (j < kNB) px[j] = __half22float2(__builtin_bit_cast(__half2, w[e]));
+      }
+    } else {
+      const f32x4_t v = raw_buffer_load_f32x4(rsrc, voff + (uint32_t)(q * 16), 0, 0);
+      const float w[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+      for (int e = 0; e < kPer; ++e) {
+        const int j = q * kPer + e;
+        if (j < kNB) px[j] = make_float2(w[2 * e], w[2 * e + 1]);
+      }
+    }
   }
 }
 
@@ -176,21 +226,36 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
   }
   const uint32_t rowStride = (uint32_t)(a.xSize * kPix);
   const uint32_t base = (uint32_t)(minY * a.xSize + minX) * kPix;
+  const uint32_t sliceBytes = (uint32_t)(a.sliceElems * kPix);
   float w = 0.f, vr = 0.f, vi = 0.f;
 #pragma unroll
   for (int ii = 0; ii < kNB; ++ii) {
-    const uint32_t rowOff = base + ii * rowStride;
+    // the whole candidate row at once; candidates outside the blob or the slice weigh 0 and,
+    // in the exact strategy, also have their pixel replaced by 0, so they add exactly +0
+    // (never -0 or NaN) whatever the pixel values are
+    // (footprints wider than 4 keep one load per candidate: the wide rows of a 5- or 6-wide
+    // box would not fit the register budget of 16 wavefronts per CU)
+    constexpr bool kWide = kNB <= 4;
+    float2 px[kNB];
+    if constexpr (kWide) load_row<kHalf, kNB>(rsrc, base + ii * rowStride, sliceBytes, px);
 #pragma unroll
     for (int jj = 0; jj < kNB; ++jj) {
       const float d2 = xd2[jj] + yz[ii];
       float wb;
-      uint32_t voff;
+      float2 f;
+      if constexpr (kWide) {
+        f = px[jj];
+      } else {
+        // skipped candidates of the exact strategy read past the slice end (zero)
+        const bool inb = kFastMath && kMode == kModeLookup && kTableLds ? true : d2 <= r2;
+        f = load_pixel<kHalf>(rsrc, inb ? base + ii * rowStride + (uint32_t)(jj * kPix)
+                                        : kOutOfRange);
+      }
       if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
         // +inf and every d2 past the blob edge map to the zero entry
         int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
         aux = aux < nTable ? aux : nTable;
         wb = tbl[aux];
-        voff = rowOff;
       } else {
         const bool in = d2 <= r2;
         if constexpr (kMode == kModeLookup) {
@@ -205,12 +270,10 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
         } else {
           wb = in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
         }
-        voff = in ? rowOff : kOutOfRange;
+        f.x = in ? f.x : 0.f;
+        f.y = in ? f.y : 0.f;
       }
       const float weight = kUnitWeight ? wb : wb * dw;
-      // the column offset travels in the instruction's scalar offset; the range check on the
-      // vector offset alone still sends skipped candidates out of range
-      const float2 f = load_pixel<kHalf>(rsrc, voff, (uint32_t)(jj * kPix));
       w += weight;
       if constexpr (kFastMath) {
         vr = __builtin_fmaf(f.x, weight, vr);
