@@ -84,16 +84,54 @@ __device__ __forceinline__ float2 load_pixel(__amdgpu_buffer_rsrc_t rsrc, uint32
 // 2 FP32 pixels each).  A load that starts past the slice end reads zeros.  Only the slice's
 // last row can have a load that straddles the end; such a row is read pixel by pixel, so no
 // result depends on how the hardware range-checks a partially valid access.
+// A slice pixel as loaded: complex float, or complex half packed in one dword (converted at
+// the multiply-add, which the compiler fuses into a mixed-precision FMA).
+template <bool kHalf> struct Pixel;
+template <> struct Pixel<false> {
+  float2 v;
+  __device__ __forceinline__ float re() const { return v.x; }
+  __device__ __forceinline__ float im() const { return v.y; }
+};
+template <> struct Pixel<true> {
+  uint32_t v;
+  __device__ __forceinline__ float re() const
+  {
+    return __half2float(__builtin_bit_cast(__half, (uint16_t)(v & 0xffffu)));
+  }
+  __device__ __forceinline__ float im() const
+  {
+    return __half2float(__builtin_bit_cast(__half, (uint16_t)(v >> 16)));
+  }
+};
+
+template <bool kHalf>
+__device__ __forceinline__ Pixel<kHalf> load_pixel_raw(__amdgpu_buffer_rsrc_t rsrc,
+                                                       uint32_t byteOffset)
+{
+  Pixel<kHalf> p;
+  if constexpr (kHalf) {
+    p.v = raw_buffer_load_u32(rsrc, byteOffset, 0, 0);
+  } else {
+    const f32x2_t v = raw_buffer_load_f32x2(rsrc, byteOffset, 0, 0);
+    p.v = make_float2(v.x, v.y);
+  }
+  return p;
+}
+
+// kNB consecutive pixels of a slice row starting at byte `voff`, with 16-byte loads (4 FP16 or
+// 2 FP32 pixels each).  A load that starts past the slice end reads zeros.  Only the slice's
+// last row can have a load that straddles the end; such a row is read pixel by pixel, so no
+// result depends on how the hardware range-checks a partially valid access.
 template <bool kHalf, int kNB>
 __device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff,
-                                         uint32_t sliceBytes, float2 (&px)[kNB])
+                                         uint32_t sliceBytes, Pixel<kHalf> (&px)[kNB])
 {
   constexpr int kPix = kHalf ? 4 : 8;
   constexpr int kPer = 16 / kPix;                  // pixels per 16-byte load
   constexpr int kLoads = (kNB + kPer - 1) / kPer;
   if (voff < sliceBytes && voff + (uint32_t)(kLoads * 16) > sliceBytes) {
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) px[j] = load_pixel<kHalf>(rsrc, voff + (uint32_t)(j * kPix));
+    for (int j = 0; j < kNB; ++j) px[j] = load_pixel_raw<kHalf>(rsrc, voff + (uint32_t)(j * kPix));
     return;
   }
   // (vector elements are named, never indexed by a loop variable: this compiler folded
@@ -106,7 +144,7 @@ __device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t rsrc, uint32_t v
 #pragma unroll
       for (int e = 0; e < kPer; ++e) {
         const int j = q * kPer + e;
-        if (j < kNB) px[j] = __half22float2(__builtin_bit_cast(__half2, w[e]));
+        if (j < kNB) px[j].v = w[e];
       }
     } else {
       const f32x4_t v = raw_buffer_load_f32x4(rsrc, voff + (uint32_t)(q * 16), 0, 0);
@@ -114,7 +152,7 @@ __device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t rsrc, uint32_t v
 #pragma unroll
       for (int e = 0; e < kPer; ++e) {
         const int j = q * kPer + e;
-        if (j < kNB) px[j] = make_float2(w[2 * e], w[2 * e + 1]);
+        if (j < kNB) px[j].v = make_float2(w[2 * e], w[2 * e + 1]);
       }
     }
   }
@@ -236,21 +274,22 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
     // (footprints wider than 4 keep one load per candidate: the wide rows of a 5- or 6-wide
     // box would not fit the register budget of 16 wavefronts per CU)
     constexpr bool kWide = kNB <= 4;
-    float2 px[kNB];
+    Pixel<kHalf> px[kNB];
     if constexpr (kWide) load_row<kHalf, kNB>(rsrc, base + ii * rowStride, sliceBytes, px);
 #pragma unroll
     for (int jj = 0; jj < kNB; ++jj) {
       const float d2 = xd2[jj] + yz[ii];
       float wb;
-      float2 f;
+      Pixel<kHalf> pix;
       if constexpr (kWide) {
-        f = px[jj];
+        pix = px[jj];
       } else {
         // skipped candidates of the exact strategy read past the slice end (zero)
         const bool inb = kFastMath && kMode == kModeLookup && kTableLds ? true : d2 <= r2;
-        f = load_pixel<kHalf>(rsrc, inb ? base + ii * rowStride + (uint32_t)(jj * kPix)
-                                        : kOutOfRange);
+        pix = load_pixel_raw<kHalf>(rsrc, inb ? base + ii * rowStride + (uint32_t)(jj * kPix)
+                                              : kOutOfRange);
       }
+      float fre = pix.re(), fim = pix.im();
       if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
         // +inf and every d2 past the blob edge map to the zero entry
         int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
@@ -270,17 +309,17 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
         } else {
           wb = in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
         }
-        f.x = in ? f.x : 0.f;
-        f.y = in ? f.y : 0.f;
+        fre = in ? fre : 0.f;
+        fim = in ? fim : 0.f;
       }
       const float weight = kUnitWeight ? wb : wb * dw;
       w += weight;
       if constexpr (kFastMath) {
-        vr = __builtin_fmaf(f.x, weight, vr);
-        vi = __builtin_fmaf(f.y, weight, vi);
+        vr = __builtin_fmaf(fre, weight, vr);
+        vi = __builtin_fmaf(fim, weight, vi);
       } else {
-        vr += f.x * weight;
-        vi += f.y * weight;
+        vr += fre * weight;
+        vi += fim * weight;
       }
     }
   }
@@ -397,11 +436,10 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrJob job0, FrLaunc
     const bool colIn = x < a.vx && y < a.vy;
     const int nz = min(depth, a.vz - bz);
     const float px = (float)(x - cxi), py = (float)(y - cyi);
-#pragma unroll
-    for (int k = 0; k < kEdge; ++k) {
+    for (int k = 0; k < nz; ++k) {
       const int li = k * 64 + lane;
       float vr = 0.f, vi = 0.f, vw = 0.f;
-      if (!split && colIn && k < nz) {
+      if (!split && colIn) {
         const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
         const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
         vr = t.x;
@@ -451,22 +489,18 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrJob job0, FrLaunc
         const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
                     t22 = sp.transformInv[2][2];
 
-        // phase 1: slab test of the column's 8 voxels.  qz = (t20 px + t21 py) + t22 pz is the
-        // reference's left-to-right sum, so the shared partial sum changes no bit.
+        // phase 1: slab test of the column's nz voxels (nz is wave-uniform: shallow bricks
+        // run fewer levels).  qz = (t20 px + t21 py) + t22 pz is the reference's left-to-right
+        // sum, so the shared partial sum changes no bit.
         const float cz = t20 * px + t21 * py;
-        unsigned long long m[kEdge];
         int cnt = 0;
-#pragma unroll
-        for (int k = 0; k < kEdge; ++k) {
+        for (int k = 0; k < nz; ++k) {
           const float pz = (float)(bz + k - cyi);
           const float qz = cz + t22 * pz;
-          const bool ok = colIn && k < nz && !(qz * qz > slab2);
-          m[k] = __ballot(ok);
-        }
-#pragma unroll
-        for (int k = 0; k < kEdge; ++k) {
-          if ((m[k] >> lane) & 1ull) ws.queue[cnt + (int)lanes_below(m[k])] = (uint16_t)(k * 64 + lane);
-          cnt += __popcll(m[k]);
+          const bool ok = colIn && !(qz * qz > slab2);
+          const unsigned long long m = __ballot(ok);
+          if (ok) ws.queue[cnt + (int)lanes_below(m)] = (uint16_t)(k * 64 + lane);
+          cnt += __popcll(m);
         }
         if (cnt == 0) continue;
         wave_sync();
@@ -536,9 +570,8 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrJob job0, FrLaunc
         wave_sync();
       }
     }
-#pragma unroll
-    for (int k = 0; k < kEdge; ++k) {
-      if (colIn && k < nz) {
+    for (int k = 0; k < nz; ++k) {
+      if (colIn) {
         const int li = k * 64 + lane;
         const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
         if (split) {
