@@ -380,27 +380,33 @@ def run_serving(args):
     print(f"# capacity {capacity:.2f} jobs/s, makespan {makespan:.2f}s", file=sys.stderr,
           flush=True)
     rates = []
-    for frac in [float(x) for x in args.rates.split(",")]:
-        lam = frac * capacity
-        rng = np.random.default_rng(99)
-        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=n_jobs))
-        arrivals -= arrivals[0]
-        jobs = fresh()
-        for j, a in zip(jobs, arrivals):
-            j.arrival = float(a)
-        t = time.perf_counter()
-        done = srv.run(jobs, realtime=True)
-        span = time.perf_counter() - t
-        lat = np.array([j.done - j.arrival for j in done])
-        stretch = np.array([(j.done - j.arrival) / iso[j.job_id] for j in done])
-        rates.append({"rate_jobs_per_s": lam, "fraction_of_capacity": frac,
-                      "throughput_gvoxel_updates_per_s": updates / span / 1e9,
-                      "latency_p50_s": float(np.percentile(lat, 50)),
-                      "latency_p99_s": float(np.percentile(lat, 99)),
-                      "stretch_p50": float(np.percentile(stretch, 50)),
-                      "stretch_p99": float(np.percentile(stretch, 99)),
-                      "span_s": span})
-        print(f"# rate {frac:.2f}: {json.dumps(rates[-1])}", file=sys.stderr, flush=True)
+    for policy in ("srpt", "fair"):
+        srv_p = S.Server(chunk=chunk, hbm_budget=args.hbm_budget, strategy=strategy,
+                         policy=policy, depth=2)
+        for frac in [float(x) for x in args.rates.split(",")]:
+            lam = frac * capacity
+            rng = np.random.default_rng(99)
+            arrivals = np.cumsum(rng.exponential(1.0 / lam, size=n_jobs))
+            arrivals -= arrivals[0]
+            jobs = fresh()
+            for j, a in zip(jobs, arrivals):
+                j.arrival = float(a)
+            t = time.perf_counter()
+            done = srv_p.run(jobs, realtime=True)
+            span = time.perf_counter() - t
+            lat = np.array([j.done - j.arrival for j in done])
+            stretch = np.array([(j.done - j.arrival) / iso[j.job_id] for j in done])
+            rates.append({"policy": policy, "rate_jobs_per_s": lam, "fraction_of_capacity": frac,
+                          "throughput_gvoxel_updates_per_s": updates / span / 1e9,
+                          "latency_p50_s": float(np.percentile(lat, 50)),
+                          "latency_p99_s": float(np.percentile(lat, 99)),
+                          "latency_p99_over_isolated_p99": float(
+                              np.percentile(lat, 99) / np.percentile(list(iso.values()), 99)),
+                          "stretch_p50": float(np.percentile(stretch, 50)),
+                          "stretch_p99": float(np.percentile(stretch, 99)),
+                          "span_s": span})
+            print(f"# {policy} rate {frac:.2f}: {json.dumps(rates[-1])}", file=sys.stderr,
+                  flush=True)
     iso_arr = np.array(list(iso.values()))
     result = {
         "metric": "GVoxel-updates/s + HBM GB/s (box 256); p50/p99 job latency, batched vs plain",

@@ -49,10 +49,12 @@ def solo(job_host, strategy, chunk=64):
     return job.volume.cpu().numpy(), job.weights.cpu().numpy()
 
 
-def test_exact_alone_equals_mix_and_oracle():
+@pytest.mark.parametrize("policy", ["srpt", "fair"])
+def test_exact_alone_equals_mix_and_oracle(policy):
     chunk = 64
     mix = make_jobs(chunk=chunk)
-    srv = S.Server(chunk=chunk, strategy=F.FR_EXACT, depth=3)
+    srv = S.Server(chunk=chunk, strategy=F.FR_EXACT, depth=3, policy=policy,
+                   tick_work=2 * 64 * 33 ** 2)
     done = srv.run([j for j, _ in mix], realtime=True)
     assert len(done) == len(SPECS)
     alone = make_jobs(chunk=chunk)

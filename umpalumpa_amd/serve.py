@@ -8,7 +8,10 @@ persistent launch per group of jobs sharing settings, so jobs that arrive while 
 inserted by the same launches.  A job's voxels receive its spaces in index order whatever else
 is in flight, so its result is the same alone and in the mix (bit for bit under the exact
 strategy).  Ticks are enqueued ahead of the GPU (up to ``depth`` in flight) and completions
-are noticed through stream tickets, so the host loop never waits on an idle GPU.
+are noticed through stream tickets, so the host loop never waits on an idle GPU.  By default
+each tick serves the admitted jobs with the least remaining work first, up to a work quantum
+that fills the GPU (shortest-remaining-work first over chunks), which keeps short jobs' latency
+near their isolated runtime when long jobs share the GPU.
 
 The reference has no serving layer; its harness runs one reconstruction at a time
 (examples/fourier_reconstruction/fr.cpp:9-76).
@@ -46,11 +49,25 @@ class Job:
     _chunk_spaces: list = field(default_factory=list)
     _next_chunk: int = 0
     _last_ticket: int = 0
+    _order: int | None = None
+
+    def _k(self) -> int:
+        if self._order is None:
+            self._order = len(F.symmetry_operators(self.symmetry))
+        return self._order
+
+    def chunk_cost(self, c: int, chunk: int) -> float:
+        """Relative insertion work of chunk c: spaces x slice area."""
+        m = min(chunk, self.n_slices - c * chunk)
+        return float(self._k() * m * (self.box + 1) ** 2)
+
+    def remaining_cost(self, chunk: int) -> float:
+        m = max(0, self.n_slices - self._next_chunk * chunk)
+        return float(self._k() * m * (self.box + 1) ** 2)
 
     def footprint(self, chunk: int) -> int:
         """Device bytes the job holds while admitted."""
-        k = len(F.symmetry_operators(self.symmetry))
-        return 12 * (self.box + 1) ** 3 + 112 * self.n_slices * k + 4 * 10000
+        return 12 * (self.box + 1) ** 3 + 112 * self.n_slices * self._k() + 4 * 10000
 
 
 class Server:
@@ -58,7 +75,7 @@ class Server:
 
     def __init__(self, device: int = 0, chunk: int = 512, hbm_budget: float = 200e9,
                  strategy: int = F.FR_FAST_MATH, depth: int = 3, alpha: float = 15.0,
-                 max_active: int = 64):
+                 max_active: int = 64, policy: str = "srpt", tick_work: float = 4 * 512 * 257 ** 2):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda", device)
@@ -68,6 +85,13 @@ class Server:
         self.depth = depth
         self.alpha = alpha
         self.max_active = max_active
+        # "srpt": each tick takes the admitted jobs with the least remaining work first, until
+        # the tick holds `tick_work` (enough to fill the GPU), so short jobs overtake long ones;
+        # "fair": every admitted job advances by one chunk per tick
+        if policy not in ("srpt", "fair"):
+            raise ValueError(policy)
+        self.policy = policy
+        self.tick_work = tick_work
         self.batch = F.FRBatch(device)
         self._tables = {}
         self.used = 0
@@ -148,6 +172,15 @@ class Server:
                     self.used -= j.footprint(self.chunk)
                     finished.append(j)
             ready = [j for j in active if j._next_chunk < len(j._chunk_spaces)]
+            if ready and self.policy == "srpt":
+                ready.sort(key=lambda j: (j.remaining_cost(self.chunk), j.arrival))
+                picked, work = [], 0.0
+                for j in ready:
+                    if picked and work >= self.tick_work:
+                        break
+                    picked.append(j)
+                    work += j.chunk_cost(j._next_chunk, self.chunk)
+                ready = picked
             if ready and len(inflight) < self.depth:
                 bjobs, ends = [], []
                 for j in ready:
