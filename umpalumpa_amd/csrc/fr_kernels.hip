@@ -1,23 +1,22 @@
 // FR insertion kernels for gfx950: voxel-owned gather.
 //
 // The reference scatters each rotated slice into the volume with three float atomics per voxel
-// (fr_cuda_kernels.cu:266-365).  Here a wavefront owns a brick of 8 x 8 x 8 voxels (a z column
-// of 8 per lane) for the whole launch and keeps their running (complex, weight) sums in its LDS
-// slot.  It culls the traverse spaces against the brick 64 at a time with one ballot and walks
+// (fr_cuda_kernels.cu:266-365).  Here a wavefront owns a brick of 8 x 8 x D voxels (a z column
+// of D per lane, D = 8, 4, 2 or 1 by distance from the centre) for the whole work item and
+// keeps their running (complex, weight) sums in its LDS slot.  It culls the traverse spaces against the brick 64 at a time with one ballot and walks
 // the surviving spaces in index order, so every voxel receives its per-space contributions in
 // the order of the reference CPU loop over spaces (fr_cpu_kernels.hpp:254): no atomics, and no
 // dependence on how bricks are scheduled.
 //
 // Per (brick, space) pair the wave
-//   1. tests the reference's slab condition (q.z^2 <= r^2) for its 512 voxels, 4 VALU each,
-//      and compacts the voxels that pass (about a quarter) into its LDS queue;
+//   1. tests the reference's slab condition (q.z^2 <= r^2) for its voxels, 4 VALU each, and
+//      compacts the voxels that pass into its LDS queue;
 //   2. runs the rest of the reference predicate and the gather over the queue with full lanes:
-//      a branch-free NB x NB candidate loop in which a candidate the reference skips reads
-//      the zero entry appended to the LDS blob table and an out-of-range buffer offset that the
-//      hardware returns as zero, adding exactly +0, so the sums are bit-identical to the
-//      reference's skip-the-pixel loop while all table and pixel loads of an item issue back
-//      to back; each voxel appears once per space, so its lane adds the contribution to the
-//      voxel's LDS sum directly.
+//      a branch-free NB x NB candidate loop over whole pixel rows in which a candidate the
+//      reference skips reads the zero entry appended to the LDS blob table, adding +-0 (finite
+//      slice values), so the sums are bit-identical to the reference's skip-the-pixel loop
+//      while all table and pixel loads of an item issue back to back; each voxel appears once
+//      per space, so its lane adds the contribution to the voxel's LDS sum directly.
 // Workgroups are persistent (one per CU: 16 wavefronts sharing the blob table in LDS) and take
 // bricks from a counter, most central (heaviest) first.
 #include "fr_device.hpp"
@@ -224,11 +223,12 @@ __device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buff
 // Branch-free gather of one voxel-update over the NB x NB candidate box around (qx, qy).
 //
 // Exact strategy: candidates the reference skips (outside the blob, outside the slice) read
-// table[nTable] = 0 and an out-of-range pixel (= 0), adding exactly +0; the running sums never
-// hold -0, so the float results equal the reference's whatever the slice values are.
-// Fast-math strategy (finite slice values): the table index is clamped into the zero entry,
-// so out-of-blob candidates weigh 0 without a test; rows and columns outside the slice still
-// read 0 through the bounds-checked buffer.
+// table[nTable] = 0, so with finite slice values they add +-0; the running sums start at +0
+// and never hold -0, so the float results equal the reference's (footprints wider than 4 read
+// skipped pixels past the buffer end, as zeros).  Fast-math strategy (finite slice values):
+// the table index is clamped into the zero entry, so out-of-blob candidates weigh 0 without a
+// test.  Rows are read whole; pixels past the slice end read 0 through the bounds-checked
+// buffer.
 // Returns false when the footprint is wider than NB.
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds>
 __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
@@ -268,9 +268,10 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
   float w = 0.f, vr = 0.f, vi = 0.f;
 #pragma unroll
   for (int ii = 0; ii < kNB; ++ii) {
-    // the whole candidate row at once; candidates outside the blob or the slice weigh 0 and,
-    // in the exact strategy, also have their pixel replaced by 0, so they add exactly +0
-    // (never -0 or NaN) whatever the pixel values are
+    // the whole candidate row at once; candidates outside the blob or the slice weigh exactly
+    // 0 (the table's zero entry), so with finite slice values they add +-0, which changes no
+    // running sum (the sums start at +0 and are never -0): the result equals the reference's
+    // skip-the-pixel loop bit for bit
     // (footprints wider than 4 keep one load per candidate: the wide rows of a 5- or 6-wide
     // box would not fit the register budget of 16 wavefronts per CU)
     constexpr bool kWide = kNB <= 4;
@@ -309,8 +310,12 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
         } else {
           wb = in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
         }
-        fre = in ? fre : 0.f;
-        fim = in ? fim : 0.f;
+        if constexpr (!kWide) {
+          // (wide rows: a skipped candidate's weight is exactly 0 and its pixel finite, so its
+          // product is +-0 and leaves every running sum, which is never -0, unchanged)
+          fre = in ? fre : 0.f;
+          fim = in ? fim : 0.f;
+        }
       }
       const float weight = kUnitWeight ? wb : wb * dw;
       w += weight;
@@ -332,14 +337,27 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
 // The reference's per-voxel predicate for one (voxel, space) pair in blob mode
 // (ProcessVoxelBlob, fr_cpu_kernels.hpp:83-102) plus membership in the reference traversal of
 // the space.  (qx, qy, zSqr) are returned for the gather, qy already shifted to slice rows.
+// With kFma the rotation is evaluated with fused multiply-adds (fewer instructions; q moves by
+// an ulp, which changes which nearly-zero-weight candidates at the blob edge are counted).  The
+// kernels use the reference's separate products: fused, the weights of small volumes drift to
+// about 1e-5 of FRCPU's, the edge of the fast-math strategy's tolerance.
+template <bool kFma>
 __device__ __forceinline__ bool blob_predicate(const umpa_traverse_space &s, const umpa_fr_constants &c,
                                                int x, int y, int z, float px, float py, float pz,
                                                float dsq, float &qx, float &qy, float &zSqr)
 {
   const float r = c.cBlobRadius;
-  qx = s.transformInv[0][0] * px + s.transformInv[0][1] * py + s.transformInv[0][2] * pz;
-  qy = s.transformInv[1][0] * px + s.transformInv[1][1] * py + s.transformInv[1][2] * pz;
-  const float qz = s.transformInv[2][0] * px + s.transformInv[2][1] * py + s.transformInv[2][2] * pz;
+  const float(*t)[3] = s.transformInv;
+  float qz;
+  if constexpr (kFma) {
+    qx = __builtin_fmaf(t[0][2], pz, __builtin_fmaf(t[0][1], py, t[0][0] * px));
+    qy = __builtin_fmaf(t[1][2], pz, __builtin_fmaf(t[1][1], py, t[1][0] * px));
+    qz = __builtin_fmaf(t[2][2], pz, __builtin_fmaf(t[2][1], py, t[2][0] * px));
+  } else {
+    qx = t[0][0] * px + t[0][1] * py + t[0][2] * pz;
+    qy = t[1][0] * px + t[1][1] * py + t[1][2] * pz;
+    qz = t[2][0] * px + t[2][1] * py + t[2][2] * pz;
+  }
   zSqr = qz * qz;
   bool pass = !(dsq > s.maxDistanceSqr) && !(qx < -r) && !(zSqr > r * r);
   qy += (float)(c.cMaxVolumeIndexYZ / 2);
@@ -523,7 +541,7 @@ __global__ __launch_bounds__(kThreads) void fr_insert_kernel(FrJob job0, FrLaunc
             zs = 0.f;
             hit = act && !(dsq > sp.maxDistanceSqr) && !(qx < 0.f) && visited_nearest(sp, xv, yv, zv);
           } else {
-            hit = act && blob_predicate(sp, c, xv, yv, zv, pxv, pyv, pzv, dsq, qx, qy, zs);
+            hit = act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, dsq, qx, qy, zs);
           }
           if (hit) {
             Contribution o;
