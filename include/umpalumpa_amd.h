@@ -310,6 +310,35 @@ int umpa_fp_synchronize(umpa_fp_op *op);
 int umpa_fp_cleanup(umpa_fp_op *op);
 void umpa_fp_destroy(umpa_fp_op *op);
 
+/* ---- FFT: the AFFT operation (fourier_transformation/afft.hpp), out-of-place, via hipFFT -- */
+
+#define UMPA_FFT_FORWARD 0 /* Direction::kForward: real -> Hermitian half (x/2+1) complex */
+#define UMPA_FFT_INVERSE 1 /* Direction::kInverse: complex -> real, unnormalised (may
+                              overwrite its input, as cuFFT's C2R does) */
+#define UMPA_FFT_F32 0     /* float / complex float */
+#define UMPA_FFT_F64 1     /* double / complex double */
+
+/* spatial: image size (x, y, z; n images), contiguous; the frequency side holds x/2+1 */
+typedef struct umpa_fft_desc {
+  umpa_size spatial;
+  int32_t direction;
+  int32_t precision;
+} umpa_fft_desc;
+
+typedef struct umpa_fft_op umpa_fft_op;
+
+/* new FFT operation (FFTCUDA, fft_cuda.cpp:107-116) on `device` and `stream` */
+int umpa_fft_create(umpa_fft_op **op, int device, void *stream);
+/* AFFT::Init (afft.hpp:40-57, fft_cuda.cpp:51-57): plans the batched transform; element count
+ * must fit an int (fft_cuda.cpp:70-75) */
+int umpa_fft_init(umpa_fft_op *op, const umpa_fft_desc *desc);
+/* AFFT::Execute (fft_cuda.cpp:59-68): `count` images (at most n of Init), device memory;
+ * in-place transforms (in == out) are not offered */
+int umpa_fft_execute(umpa_fft_op *op, const void *in, void *out, uint64_t count);
+int umpa_fft_synchronize(umpa_fft_op *op);
+int umpa_fft_cleanup(umpa_fft_op *op);
+void umpa_fft_destroy(umpa_fft_op *op);
+
 /* ---- misc --------------------------------------------------------------------------------- */
 const char *umpa_status_string(int status);
 /* library version, for the loader's sanity check */

@@ -1,6 +1,7 @@
-// The reference's FR and FP operation tests (tests/operations/fourier_reconstruction/
+// The reference's FR, FP and FFT operation tests (tests/operations/fourier_reconstruction/
 // fr_tests.hpp + afr_common.hpp, tests/operations/fourier_processing/fp_tests.hpp +
-// afp_common.hpp), written against this library's C++ API with FRHIP / FPHIP as the backend and
+// afp_common.hpp, tests/operations/fourier_transformation/fft_tests.hpp + afft_common.hpp),
+// written against this library's C++ API with FRHIP / FPHIP / FFTHIP as the backend and
 // managed memory as the payload storage (as the reference's CUDA test fixtures use).  gtest is
 // not available, so a minimal runner with the same assertion semantics is used
 // (ASSERT_FLOAT_EQ = within 4 ulps).
@@ -14,11 +15,13 @@
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
 #include "umpalumpa_amd/operations/fourier_processing/afp.hpp"
 #include "umpalumpa_amd/operations/fourier_reconstruction/afr.hpp"
+#include "umpalumpa_amd/operations/fourier_transformation/afft.hpp"
 
 using namespace umpalumpa;
 using namespace umpalumpa::data;
@@ -56,6 +59,14 @@ static bool float_eq(float a, float b)
     if (!float_eq((a), (b))) {                                                               \
       std::printf("  %s:%d: ASSERT_FLOAT_EQ(%s, %s) failed: %.9g vs %.9g\n", __FILE__,      \
                   __LINE__, #a, #b, (double)(a), (double)(b));                               \
+      throw Failure{};                                                                       \
+    }                                                                                        \
+  } while (0)
+#define ASSERT_NEAR(a, b, d)                                                                   \
+  do {                                                                                       \
+    if (!(std::abs((double)(a) - (double)(b)) <= (double)(d))) {                             \
+      std::printf("  %s:%d: ASSERT_NEAR(%s, %s) failed: %.9g vs %.9g\n", __FILE__, __LINE__, \
+                  #a, #b, (double)(a), (double)(b));                                         \
       throw Failure{};                                                                       \
     }                                                                                        \
   } while (0)
@@ -510,6 +521,158 @@ TEST(ShiftOdd)
   RunFP(s, Size(4, 3, 1, 1), Size(4, 3, 1, 1));
 }
 }  // namespace fp
+
+// ======================= FFT (fft_tests.hpp / afft_common.hpp) ============================
+namespace fft {
+using namespace umpalumpa::fourier_transformation;
+
+struct Fixture {
+  Managed mem;
+  Payload<FourierDescriptor> spatial, frequency;
+  explicit Fixture(const Size &s)
+    : spatial(FourierDescriptor(s), mem.Create(FourierDescriptor(s).Elems() * 4,
+                                               DataType::Get<float>())),
+      frequency(FourierDescriptor(s, PaddingDescriptor(), FourierDescriptor::FourierSpaceDescriptor{}),
+        mem.Create(FourierDescriptor(s, PaddingDescriptor(), FourierDescriptor::FourierSpaceDescriptor{})
+                       .Elems() * 8,
+          DataType::Get<std::complex<float>>()))
+  {}
+};
+
+// images [offset, offset + count) of a payload, as the reference's Payload::Subset gives them
+static Payload<FourierDescriptor> Subset(const Payload<FourierDescriptor> &p, size_t offset,
+  size_t count)
+{
+  const Size &s = p.info.GetSpatialSize();
+  Size sub(s.x, s.y, s.z, std::min(count, s.n - offset));
+  auto fsd = p.info.GetFourierSpaceDescriptor();
+  FourierDescriptor ld = fsd ? FourierDescriptor(sub, PaddingDescriptor(), *fsd) : FourierDescriptor(sub);
+  const size_t elem = p.dataInfo.GetType().GetSize();
+  auto *base = static_cast<char *>(p.GetPtr()) + offset * p.info.GetPaddedSize().single * elem;
+  return Payload(ld, PhysicalDescriptor(base, ld.Elems() * elem, p.dataInfo.GetType(),
+                       ManagedBy::Manually, nullptr));
+}
+
+// impulse at the origin of every image -> constant 1 + 0i spectrum (afft_common.hpp:70-101)
+TEST(InpulseOriginForward)
+{
+  Fixture fx(Size(5, 5, 1, 1));
+  auto *in = static_cast<float *>(fx.spatial.GetPtr());
+  in[0] = 1.f;
+  FFTHIP op;
+  auto o = AFFT::OutputData(fx.frequency);
+  auto i = AFFT::InputData(fx.spatial);
+  ASSERT_TRUE(op.Init(o, i, Settings(Locality::kOutOfPlace, Direction::kForward)));
+  ASSERT_TRUE(op.Execute(o, i));
+  op.Synchronize();
+  auto *out = static_cast<std::complex<float> *>(fx.frequency.GetPtr());
+  for (size_t k = 0; k < fx.frequency.info.GetPaddedSize().total; ++k) {
+    ASSERT_NEAR(1, out[k].real(), 1e-5);
+    ASSERT_NEAR(0, out[k].imag(), 1e-5);
+  }
+}
+
+// constant spectrum -> impulse at the origin, unnormalised (afft_common.hpp:103-150)
+TEST(InpulseOriginInverse)
+{
+  Fixture fx(Size(5, 5, 1, 1));
+  auto *in = static_cast<std::complex<float> *>(fx.frequency.GetPtr());
+  for (size_t k = 0; k < fx.frequency.info.GetPaddedSize().single; ++k) in[k] = { 1.f, 0.f };
+  FFTHIP op;
+  auto o = AFFT::OutputData(fx.spatial);
+  auto i = AFFT::InputData(fx.frequency);
+  ASSERT_TRUE(op.Init(o, i, Settings(Locality::kOutOfPlace, Direction::kInverse)));
+  ASSERT_TRUE(op.Execute(o, i));
+  op.Synchronize();
+  auto *out = static_cast<float *>(fx.spatial.GetPtr());
+  const float single = (float)fx.spatial.info.GetSize().single;
+  for (size_t k = 0; k < fx.spatial.info.GetSize().total; ++k)
+    ASSERT_NEAR(k == 0 ? 1.f : 0.f, out[k] / single, 1e-5);
+}
+
+// impulse one sample off the origin -> unit magnitude everywhere (afft_common.hpp:152-181)
+TEST(InpulseShiftedForward)
+{
+  Fixture fx(Size(5, 5, 1, 1));
+  static_cast<float *>(fx.spatial.GetPtr())[1] = 1.f;
+  FFTHIP op;
+  auto o = AFFT::OutputData(fx.frequency);
+  auto i = AFFT::InputData(fx.spatial);
+  ASSERT_TRUE(op.Init(o, i, Settings(Locality::kOutOfPlace, Direction::kForward)));
+  ASSERT_TRUE(op.Execute(o, i));
+  op.Synchronize();
+  auto *out = static_cast<std::complex<float> *>(fx.frequency.GetPtr());
+  for (size_t k = 0; k < fx.frequency.info.GetPaddedSize().total; ++k)
+    ASSERT_NEAR(1.f, std::abs(out[k]), 1e-5);
+}
+
+// FFT then inverse FFT, optionally over batches of images; the normalised result equals the
+// normal-distributed input (afft_common.hpp:183-262).  The reference skips its batched
+// variants ("not yet supported"); here they run, one Init per direction then an Execute per
+// batch, the last batch shorter when the count does not divide.
+static void RunFFTIFFT(const Size &size, size_t batch)
+{
+  Fixture fx(size);
+  const size_t total = fx.spatial.info.GetPaddedSize().total;
+  std::vector<float> ref(total);
+  std::mt19937 gen(42);
+  std::normal_distribution<float> dist(0.f, 1.f);
+  for (auto &v : ref) v = dist(gen);
+  std::memcpy(fx.spatial.GetPtr(), ref.data(), total * sizeof(float));
+  FFTHIP op;
+  const Settings fwd(Locality::kOutOfPlace, Direction::kForward);
+  const size_t step = batch ? batch : size.n;
+  for (size_t off = 0; off < size.n; off += step) {
+    auto sp = Subset(fx.spatial, off, step);
+    auto fr = Subset(fx.frequency, off, step);
+    auto o = AFFT::OutputData(fr);
+    auto i = AFFT::InputData(sp);
+    if (off == 0) ASSERT_TRUE(op.Init(o, i, fwd));
+    ASSERT_TRUE(op.Execute(o, i));
+  }
+  op.Synchronize();  // the forward transforms read the buffer cleared next, on the host
+  std::memset(fx.spatial.GetPtr(), 0, total * sizeof(float));
+  for (size_t off = 0; off < size.n; off += step) {
+    auto sp = Subset(fx.spatial, off, step);
+    auto fr = Subset(fx.frequency, off, step);
+    auto o = AFFT::OutputData(sp);
+    auto i = AFFT::InputData(fr);
+    if (off == 0) ASSERT_TRUE(op.Init(o, i, fwd.CreateInverse()));
+    ASSERT_TRUE(op.Execute(o, i));
+  }
+  op.Synchronize();
+  const auto *out = static_cast<const float *>(fx.spatial.GetPtr());
+  const float norm = (float)fx.spatial.info.GetNormFactor();
+  for (size_t k = 0; k < total; ++k) ASSERT_NEAR(ref[k], out[k] * norm, 1e-5);
+}
+
+TEST(FFTIFFT) { RunFFTIFFT(Size(5, 5, 1, 15), 0); }
+TEST(FFTIFFT_Batch) { RunFFTIFFT(Size(5, 5, 1, 50), 10); }
+TEST(FFTIFFT_BatchNotDivisible) { RunFFTIFFT(Size(5, 5, 1, 53), 10); }
+TEST(FFTIFFT_Large) { RunFFTIFFT(Size(300, 200, 32, 55), 0); }
+
+// Init rejects in-place transforms and mismatched element types; Execute needs Init
+TEST(FFTContract)
+{
+  Fixture fx(Size(8, 8, 1, 2));
+  FFTHIP op;
+  auto o = AFFT::OutputData(fx.frequency);
+  auto i = AFFT::InputData(fx.spatial);
+  ASSERT_TRUE(!op.Execute(o, i));
+  ASSERT_TRUE(!op.Init(o, i, Settings(Locality::kInPlace, Direction::kForward)));
+  ASSERT_EQ(op.LastStatus(), UMPA_ERR_UNSUPPORTED);
+  // forward needs real in, complex out
+  auto wrongO = AFFT::OutputData(fx.spatial);
+  auto wrongI = AFFT::InputData(fx.frequency);
+  ASSERT_TRUE(!op.Init(wrongO, wrongI, Settings(Locality::kOutOfPlace, Direction::kForward)));
+  ASSERT_TRUE(op.Init(o, i, Settings(Locality::kOutOfPlace, Direction::kForward)));
+  // more images than at Init are not equivalent
+  Fixture big(Size(8, 8, 1, 3));
+  auto bo = AFFT::OutputData(big.frequency);
+  auto bi = AFFT::InputData(big.spatial);
+  ASSERT_TRUE(!op.Execute(bo, bi));
+}
+}  // namespace fft
 
 int main(int argc, char **argv)
 {

@@ -1,8 +1,10 @@
-"""The C++ operation API (AFR / FRHIP, AFP / FPHIP over the C ABI), tested by the C++ program
-tests/cpp/fr_fp_tests.cpp, which restates the reference's own gtest cases
+"""The C++ operation API (AFR / FRHIP, AFP / FPHIP, AFFT / FFTHIP over the C ABI), tested by the
+C++ program tests/cpp/fr_fp_tests.cpp, which restates the reference's own gtest cases
 (tests/operations/fourier_reconstruction/fr_tests.hpp: 12 plane cases with traverse-space known
 answers and the zero / non-zero pattern of the volume; tests/operations/fourier_processing/
-fp_tests.hpp: 9 crop / normalise / centre / filter / max-frequency / shift cases)."""
+fp_tests.hpp: 9 crop / normalise / centre / filter / max-frequency / shift cases;
+tests/operations/fourier_transformation/fft_tests.hpp: impulses forward / inverse / shifted and
+FFT -> IFFT round trips, batched ones included)."""
 import os
 import subprocess
 
@@ -28,8 +30,8 @@ def test_host_cases():
 
 @pytest.mark.gpu
 def test_all_cases_on_gpu():
-    """All reference FR / FP cases through FRHIP / FPHIP on the device."""
+    """All reference FR / FP / FFT cases through FRHIP / FPHIP / FFTHIP on the device."""
     r = run()
     assert r.returncode == 0, r.stdout + r.stderr
     assert " 0 failures" in r.stdout
-    assert r.stdout.count("[ OK ]") >= 25
+    assert r.stdout.count("[ OK ]") >= 34

@@ -26,6 +26,8 @@ UMPA_ERR_OUT_OF_MEMORY = 7
 FR_DYNAMIC, FR_LOOKUP = 0, 1
 FR_FAST, FR_PRECISE = 0, 1
 FR_EXACT, FR_FAST_MATH = 0, 1
+FFT_FORWARD, FFT_INVERSE = 0, 1
+FFT_F32, FFT_F64 = 0, 1
 
 # umpa_traverse_space, 112 B (fourier_reconstruction/traverse_space.hpp:14-48)
 TRAVERSE_SPACE_DTYPE = np.dtype([
@@ -78,6 +80,10 @@ class SchedParams(C.Structure):
 
     def as_tuple(self):
         return (self.depthScale, self.priority, self.maxParts, self.splitAt)
+
+
+class FFTDesc(C.Structure):
+    _fields_ = [("spatial", Size), ("direction", C.c_int32), ("precision", C.c_int32)]
 
 
 class FPSettings(C.Structure):
@@ -137,6 +143,12 @@ PROTOTYPES = [
     ("umpa_fr_batch_query", C.c_int, [_P, _U64, C.POINTER(C.c_int)]),
     ("umpa_fr_batch_synchronize", C.c_int, [_P]),
     ("umpa_fr_batch_destroy", None, [_P]),
+    ("umpa_fft_create", C.c_int, [C.POINTER(_P), C.c_int, _P]),
+    ("umpa_fft_init", C.c_int, [_P, C.POINTER(FFTDesc)]),
+    ("umpa_fft_execute", C.c_int, [_P, _P, _P, _U64]),
+    ("umpa_fft_synchronize", C.c_int, [_P]),
+    ("umpa_fft_cleanup", C.c_int, [_P]),
+    ("umpa_fft_destroy", None, [_P]),
     ("umpa_fp_create", C.c_int, [C.POINTER(_P), C.c_int, _P]),
     ("umpa_fp_init", C.c_int, [_P, C.POINTER(FPDesc), C.POINTER(FPSettings)]),
     ("umpa_fp_execute", C.c_int, [_P, _P, _U64, _P, _U64, _P]),

@@ -14,16 +14,18 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import (FR_DYNAMIC, FR_EXACT, FR_FAST, FR_FAST_MATH, FR_LOOKUP, FR_PRECISE,
-                   TRAVERSE_SPACE_DTYPE, FPDesc, FPSettings, FRConstants, FRDesc, FRJob,
-                   FRSettings, SchedParams, Size, check)
+from ._lib import (FFT_F32, FFT_F64, FFT_FORWARD, FFT_INVERSE, FR_DYNAMIC, FR_EXACT,
+                   FR_FAST, FR_FAST_MATH, FR_LOOKUP, FR_PRECISE, TRAVERSE_SPACE_DTYPE, FFTDesc,
+                   FPDesc, FPSettings, FRConstants, FRDesc, FRJob, FRSettings, SchedParams, Size,
+                   check)
 
 __all__ = [
     "FR_DYNAMIC", "FR_LOOKUP", "FR_FAST", "FR_PRECISE", "FR_EXACT", "FR_FAST_MATH",
     "TRAVERSE_SPACE_DTYPE", "FRSettings", "FRDesc", "FPSettings", "Size",
     "FourierReconstruction", "FourierProcessing", "create_constants", "fill_blob_table",
     "compute_traverse_space", "traverse_spaces", "generate_traverse_spaces", "BatchJob",
-    "FRBatch", "SchedParams", "symmetry_operators", "finalize",
+    "FRBatch", "SchedParams", "symmetry_operators", "finalize", "FourierTransform",
+    "FFT_FORWARD", "FFT_INVERSE", "FFT_F32", "FFT_F64",
 ]
 
 
@@ -327,6 +329,41 @@ class FRBatch:
     def wait(self) -> None:
         import torch
         torch.cuda.current_stream(self.device).wait_stream(self.stream)
+
+
+# ---- AFFT ---------------------------------------------------------------------------------
+class FourierTransform:
+    """AFFT on one GPU (batched out-of-place R2C forward / unnormalised C2R inverse, hipFFT)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self.device = device
+        self._stream = stream if stream is not None else torch.cuda.current_stream(device)
+        self._h = C.c_void_p()
+        check(_lib.load().umpa_fft_create(C.byref(self._h), device,
+                                          _stream_handle(self._stream)), "umpa_fft_create")
+        self.last_status = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _lib.load().umpa_fft_destroy(h)
+            self._h = None
+
+    def init(self, spatial, n: int, direction: int = FFT_FORWARD,
+             precision: int = FFT_F32) -> bool:
+        """spatial: (x, y[, z]) image size; n: images."""
+        x, y, z = (tuple(spatial) + (1, 1))[:3]
+        d = FFTDesc(Size(x, y, z, n), direction, precision)
+        self.last_status = _lib.load().umpa_fft_init(self._h, C.byref(d))
+        return self.last_status == 0
+
+    def execute(self, inp, out, count: int) -> bool:
+        self.last_status = _lib.load().umpa_fft_execute(self._h, _ptr(inp), _ptr(out), count)
+        return self.last_status == 0
+
+    def synchronize(self) -> None:
+        check(_lib.load().umpa_fft_synchronize(self._h), "synchronize")
 
 
 # ---- AFP ----------------------------------------------------------------------------------

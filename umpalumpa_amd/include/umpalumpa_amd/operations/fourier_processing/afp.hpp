@@ -12,11 +12,8 @@
 
 #include "../../data/payload.hpp"
 #include "../basic_operation.hpp"
+#include "../fourier_transformation/locality.hpp"
 #include "umpalumpa_amd.h"
-
-namespace umpalumpa::fourier_transformation {
-enum class Locality { kInPlace, kOutOfPlace };
-}
 
 namespace umpalumpa::fourier_processing {
 
