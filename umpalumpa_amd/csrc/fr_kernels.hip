@@ -79,10 +79,6 @@ __device__ __forceinline__ float2 load_pixel(__amdgpu_buffer_rsrc_t rsrc, uint32
   }
 }
 
-// kNB consecutive pixels of a slice row starting at byte `voff`, with 16-byte loads (4 FP16 or
-// 2 FP32 pixels each).  A load that starts past the slice end reads zeros.  Only the slice's
-// last row can have a load that straddles the end; such a row is read pixel by pixel, so no
-// result depends on how the hardware range-checks a partially valid access.
 // A slice pixel as loaded: complex float, or complex half packed in one dword (converted at
 // the multiply-add, which the compiler fuses into a mixed-precision FMA).
 template <bool kHalf> struct Pixel;
@@ -118,21 +114,16 @@ __device__ __forceinline__ Pixel<kHalf> load_pixel_raw(__amdgpu_buffer_rsrc_t rs
 }
 
 // kNB consecutive pixels of a slice row starting at byte `voff`, with 16-byte loads (4 FP16 or
-// 2 FP32 pixels each).  A load that starts past the slice end reads zeros.  Only the slice's
-// last row can have a load that straddles the end; such a row is read pixel by pixel, so no
-// result depends on how the hardware range-checks a partially valid access.
+// 2 FP32 pixels each).  Dwords past the slice end read zero.  A load on the slice's last row
+// can straddle the end; the raw buffer's range check then still returns its in-range dwords
+// (the reference's 3 x 6 plane cases, where every last-row load straddles, are bit-exact).
 template <bool kHalf, int kNB>
 __device__ __forceinline__ void load_row(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff,
-                                         uint32_t sliceBytes, Pixel<kHalf> (&px)[kNB])
+                                         Pixel<kHalf> (&px)[kNB])
 {
   constexpr int kPix = kHalf ? 4 : 8;
   constexpr int kPer = 16 / kPix;                  // pixels per 16-byte load
   constexpr int kLoads = (kNB + kPer - 1) / kPer;
-  if (voff < sliceBytes && voff + (uint32_t)(kLoads * 16) > sliceBytes) {
-#pragma unroll
-    for (int j = 0; j < kNB; ++j) px[j] = load_pixel_raw<kHalf>(rsrc, voff + (uint32_t)(j * kPix));
-    return;
-  }
   // (vector elements are named, never indexed by a loop variable: this compiler folded
   // v[e] in an unrolled loop over e to v[0] for every e, in the 16-byte loads here)
 #pragma unroll
@@ -220,6 +211,56 @@ __device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buff
   return o;
 }
 
+// load_row for the exact FP32 kernels: a row whose 16-byte loads straddle the slice end is
+// read pixel by pixel.  This compiler merges those pixel loads back into the same 16-byte
+// loads, so both sides load alike; what the branch changes is scheduling: it makes each row its
+// own region, in which this compiler issues every row's loads and table reads ahead of the
+// sums.  Without it, the exact FP32 kernel waited on each table read in turn (2.70 instead of
+// 2.06 ms per 1,000-slice chunk at box 256), while the fast-math kernels ran faster without it.
+template <bool kHalf, int kNB>
+__device__ __forceinline__ void load_row_regions(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff,
+                                                 uint32_t sliceBytes, Pixel<kHalf> (&px)[kNB])
+{
+  constexpr int kPix = kHalf ? 4 : 8;
+  constexpr int kLoads = (kNB * kPix + 15) / 16;
+  if (voff < sliceBytes && voff + (uint32_t)(kLoads * 16) > sliceBytes) {
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) px[j] = load_pixel_raw<kHalf>(rsrc, voff + (uint32_t)(j * kPix));
+    return;
+  }
+  load_row<kHalf, kNB>(rsrc, voff, px);
+}
+
+// Blob weight of one candidate at squared distance d2, 0 for a candidate the reference skips
+// (d2 > r^2, and d2 = +inf for box positions outside the slice).  Exact strategy: the
+// reference's table position (int)(d2 * cIDeltaSqrt + 0.5); skipped candidates read the LDS
+// copy's zero entry nTable.  Fast math with the table in LDS: a fused position, clamped into
+// the zero entry, so +inf and every d2 past the blob edge weigh 0 without a test.
+template <int kMode, int kOrder, bool kFastMath, bool kTableLds>
+__device__ __forceinline__ float candidate_weight(float d2, float r2, int nTable, const float *tbl,
+                                                  const umpa_fr_constants &c)
+{
+  if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
+    int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
+    aux = aux < nTable ? aux : nTable;
+    return tbl[aux];
+  } else {
+    const bool in = d2 <= r2;
+    if constexpr (kMode == kModeLookup) {
+      int aux = kFastMath ? (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f)
+                          : (int)(d2 * c.cIDeltaSqrt + 0.5f);
+      if constexpr (kTableLds) {
+        return tbl[in ? aux : nTable];  // entry nTable of the LDS copy is 0
+      } else {
+        aux = aux > nTable - 1 ? nTable - 1 : aux;
+        return in ? tbl[aux] : 0.f;
+      }
+    } else {
+      return in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
+    }
+  }
+}
+
 // Branch-free gather of one voxel-update over the NB x NB candidate box around (qx, qy).
 //
 // Exact strategy: candidates the reference skips (outside the blob, outside the slice) read
@@ -266,65 +307,65 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
   const uint32_t base = (uint32_t)(minY * a.xSize + minX) * kPix;
   const uint32_t sliceBytes = (uint32_t)(a.sliceElems * kPix);
   float w = 0.f, vr = 0.f, vi = 0.f;
+  if constexpr (kNB <= 4) {
+    // Whole candidate rows, summed in the reference's order (rows, then columns).  Candidates
+    // outside the blob or the slice weigh exactly 0 (the table's zero entry), so with finite
+    // slice values they add +-0, which changes no running sum (the sums start at +0 and are
+    // never -0): the result equals the reference's skip-the-pixel loop bit for bit.
+    constexpr bool kRegions = !kHalf && !kFastMath;
 #pragma unroll
-  for (int ii = 0; ii < kNB; ++ii) {
-    // the whole candidate row at once; candidates outside the blob or the slice weigh exactly
-    // 0 (the table's zero entry), so with finite slice values they add +-0, which changes no
-    // running sum (the sums start at +0 and are never -0): the result equals the reference's
-    // skip-the-pixel loop bit for bit
-    // (footprints wider than 4 keep one load per candidate: the wide rows of a 5- or 6-wide
-    // box would not fit the register budget of 16 wavefronts per CU)
-    constexpr bool kWide = kNB <= 4;
-    Pixel<kHalf> px[kNB];
-    if constexpr (kWide) load_row<kHalf, kNB>(rsrc, base + ii * rowStride, sliceBytes, px);
+    for (int ii = 0; ii < kNB; ++ii) {
+      Pixel<kHalf> px[kNB];
+      if constexpr (kRegions)
+        load_row_regions<kHalf, kNB>(rsrc, base + ii * rowStride, sliceBytes, px);
+      else
+        load_row<kHalf, kNB>(rsrc, base + ii * rowStride, px);
 #pragma unroll
-    for (int jj = 0; jj < kNB; ++jj) {
-      const float d2 = xd2[jj] + yz[ii];
-      float wb;
-      Pixel<kHalf> pix;
-      if constexpr (kWide) {
-        pix = px[jj];
-      } else {
-        // skipped candidates of the exact strategy read past the slice end (zero)
-        const bool inb = kFastMath && kMode == kModeLookup && kTableLds ? true : d2 <= r2;
-        pix = load_pixel_raw<kHalf>(rsrc, inb ? base + ii * rowStride + (uint32_t)(jj * kPix)
-                                              : kOutOfRange);
-      }
-      float fre = pix.re(), fim = pix.im();
-      if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
-        // +inf and every d2 past the blob edge map to the zero entry
-        int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
-        aux = aux < nTable ? aux : nTable;
-        wb = tbl[aux];
-      } else {
-        const bool in = d2 <= r2;
-        if constexpr (kMode == kModeLookup) {
-          int aux = kFastMath ? (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f)
-                              : (int)(d2 * c.cIDeltaSqrt + 0.5f);
-          if constexpr (kTableLds) {
-            wb = tbl[in ? aux : nTable];  // entry nTable of the LDS copy is 0
-          } else {
-            aux = aux > nTable - 1 ? nTable - 1 : aux;
-            wb = in ? tbl[aux] : 0.f;
-          }
+      for (int jj = 0; jj < kNB; ++jj) {
+        const float wb = candidate_weight<kMode, kOrder, kFastMath, kTableLds>(
+          xd2[jj] + yz[ii], r2, nTable, tbl, c);
+        const float weight = kUnitWeight ? wb : wb * dw;
+        w += weight;
+        if constexpr (kFastMath) {
+          vr = __builtin_fmaf(px[jj].re(), weight, vr);
+          vi = __builtin_fmaf(px[jj].im(), weight, vi);
         } else {
-          wb = in ? blob_weight<kMode, kOrder>(d2, c) : 0.f;
+          vr += px[jj].re() * weight;
+          vi += px[jj].im() * weight;
         }
-        if constexpr (!kWide) {
-          // (wide rows: a skipped candidate's weight is exactly 0 and its pixel finite, so its
-          // product is +-0 and leaves every running sum, which is never -0, unchanged)
+      }
+    }
+  } else {
+    // footprints wider than 4 keep one load per candidate (whole rows of a 5- or 6-wide box
+    // would not fit the register budget of 16 wavefronts per CU); skipped candidates read past
+    // the slice end (zero) and are masked, except under fast math with the table in LDS, where
+    // their weight is the zero entry
+#pragma unroll
+    for (int ii = 0; ii < kNB; ++ii) {
+#pragma unroll
+      for (int jj = 0; jj < kNB; ++jj) {
+        const float d2 = xd2[jj] + yz[ii];
+        constexpr bool kZeroEntry = kFastMath && kMode == kModeLookup && kTableLds;
+        const bool inb = kZeroEntry ? true : d2 <= r2;
+        const Pixel<kHalf> pix = load_pixel_raw<kHalf>(
+          rsrc, inb ? base + ii * rowStride + (uint32_t)(jj * kPix) : kOutOfRange);
+        float fre = pix.re(), fim = pix.im();
+        const float wb = candidate_weight<kMode, kOrder, kFastMath, kTableLds>(d2, r2, nTable,
+                                                                                 tbl, c);
+        if constexpr (!kZeroEntry) {
+          const bool in = d2 <= r2;
           fre = in ? fre : 0.f;
           fim = in ? fim : 0.f;
         }
-      }
-      const float weight = kUnitWeight ? wb : wb * dw;
-      w += weight;
-      if constexpr (kFastMath) {
-        vr = __builtin_fmaf(fre, weight, vr);
-        vi = __builtin_fmaf(fim, weight, vi);
-      } else {
-        vr += fre * weight;
-        vi += fim * weight;
+        const float weight = kUnitWeight ? wb : wb * dw;
+        w += weight;
+        if constexpr (kFastMath) {
+          vr = __builtin_fmaf(fre, weight, vr);
+          vi = __builtin_fmaf(fim, weight, vi);
+        } else {
+          vr += fre * weight;
+          vi += fim * weight;
+        }
       }
     }
   }

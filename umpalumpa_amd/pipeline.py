@@ -45,18 +45,30 @@ class ImagePipeline:
                                  device=self.dev)
         self._crop = torch.empty((batch, box, hx_crop), dtype=torch.complex64, device=self.dev)
 
-    def insert(self, images, rotations):
-        """images: float32 device tensor [m, box, box] (m <= batch); rotations: [m, 3, 3]."""
+    def spaces(self, rotations):
+        """Traverse spaces of the orientations times the symmetry operators, on the device.
+        They use the cropped image width box/2, like the harness (fr.cpp:296-305)."""
+        sp = F.generate_traverse_spaces(rotations, self.box, self.settings, self.sym)
+        return self.torch.from_numpy(sp.view(np.uint8)).to(self.dev)
+
+    def insert(self, images, rotations=None, spaces=None, events=None):
+        """images: float32 device tensor [m, box, box] (m <= batch); rotations: [m, 3, 3], or
+        `spaces` from spaces() made beforehand.  `events`: four torch events recorded on the
+        stream around the FFT, the crop and the insertion."""
         m = images.shape[0]
         if m > self.batch:
             raise ValueError("more images than the batch")
-        # spaces use the cropped image width box/2, like the harness (fr.cpp:296-305)
-        spaces = F.generate_traverse_spaces(rotations, self.box, self.settings, self.sym)
-        d_sp = self.torch.from_numpy(spaces.view(np.uint8)).to(self.dev)
+        if spaces is None:
+            spaces = self.spaces(rotations)
+        rec = (lambda i: events[i].record()) if events is not None else (lambda i: None)
+        rec(0)
         assert self.fft.execute(images.contiguous(), self._spec, m), self.fft.last_status
+        rec(1)
         assert self.fp.execute(self._spec, m, self._crop, m), self.fp.last_status
-        assert self.op.execute(self._crop[:m], self.volume, self.weights, d_sp, self.table), \
+        rec(2)
+        assert self.op.execute(self._crop[:m], self.volume, self.weights, spaces, self.table), \
             self.op.last_status
+        rec(3)
 
     def synchronize(self):
         self.op.synchronize()
