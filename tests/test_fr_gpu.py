@@ -163,6 +163,22 @@ def test_symmetry_ops_and_space_order():
     assert _bits_equal(v, ov) and _bits_equal(w, ow)
 
 
+@pytest.mark.parametrize("radius", [0.8, 1.2, 1.5, 2.2, 2.5, 3.2])
+@pytest.mark.parametrize("interp", [F.FR_LOOKUP, F.FR_DYNAMIC])
+def test_radius_sweep(radius, interp):
+    """Footprint widths 3..6 (the clamped 6 for r = 3.2) and every candidate box, including the
+    smaller box of voxels far from the slice plane: exact bit-identical to the oracle, fast
+    math within 1e-5."""
+    s = settings(interpolation=interp, radius=radius)
+    fft, spaces, desc, table = problem(40, 30, s, seed=int(radius * 10))
+    ov, ow, _ = run_oracle(s, fft, spaces, table, 40)
+    v, w, _ = run_hip(s, fft, spaces, desc, table, 40)
+    assert _bits_equal(v, ov) and _bits_equal(w, ow)
+    if interp == F.FR_LOOKUP:
+        fv, fw, _ = run_hip(s, fft, spaces, desc, table, 40, strategy=F.FR_FAST_MATH)
+        assert rel_l2(fv, ov) <= 1e-5 and rel_l2(fw, ow) <= 1e-5
+
+
 def test_accumulates_onto_existing_volume():
     s = settings()
     fft, spaces, desc, table = problem(48, 40, s, seed=11)
