@@ -19,10 +19,10 @@ oracle's restatement where that is absent) timed on one host core over one chunk
 --workload batched: 16 independent reconstructions (boxes drawn from {64, 96, 128, 192, 256,
 320}, slice counts log-uniform in 500-20,000, uniform SO(3) orientations, FP16 slices with a
 per-slice max-abs scale, FP32 accumulation) through the batch runner: one persistent launch per
-step of 1,000 slices of every job, the chunk loop replayed as one HIP graph.  A step of the
-bench is the whole batch.  The line also reports the same jobs run one after another through
-the plain FP32 operation (exact strategy), and p50/p99 job latency of both when all 16 jobs are
-submitted at once.
+step of 4,000 slices of every job (--chunk; at 20,000 the whole batch is one launch), the
+chunk loop replayed as one HIP graph.  A step of the bench is the whole batch.  The line also
+reports the same jobs run one after another through the plain FP32 operation (exact strategy,
+same chunk size), and p50/p99 job latency of both when all 16 jobs are submitted at once.
 
 --workload serving: a Poisson stream of reconstruction jobs (boxes {96, 128, 192, 256, 384},
 1,000-50,000 particles log-uniform, blob radius {1.5, 1.9, 2.5}, symmetry {C1, C4, D2}, FP16
@@ -62,7 +62,9 @@ def parse():
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--box", type=int, default=256)
-    p.add_argument("--chunk", type=int, default=1000)
+    p.add_argument("--chunk", type=int, default=None,
+                   help="slices per step: 1,000 (plain, pipeline) or 4,000 per job (batched) "
+                        "by default")
     p.add_argument("--slices", type=int, default=20000)
     p.add_argument("--strategy", default="fast", choices=["exact", "fast", "f16"],
                    help="fast: FP32 with fused multiply-add, within 1e-5 rel-L2 of FRCPU (the "
@@ -179,7 +181,7 @@ def run_batched(args):
     import torch
     from umpalumpa_amd import fr as F
     dev = torch.device("cuda:0")
-    chunk = args.chunk
+    chunk = args.chunk or 4000
     s = F.FRSettings(F.FR_LOOKUP, 0, F.FR_PRECISE, 15.0, 1.9)
     specs = batched_workload(args.jobs, 2024)
     t0 = time.time()
@@ -442,7 +444,7 @@ def run_pipeline(args):
     from umpalumpa_amd import fr as F
     from umpalumpa_amd.pipeline import ImagePipeline
     dev = torch.device("cuda:0")
-    box, chunk = args.box, args.chunk
+    box, chunk = args.box, args.chunk or 1000
     s = F.FRSettings(F.FR_LOOKUP, 0, F.FR_PRECISE, 15.0, 1.9)
     strategy = F.FR_FAST_MATH if args.strategy == "fast" else F.FR_EXACT
     t0 = time.time()
@@ -604,7 +606,7 @@ def main():
     from umpalumpa_amd import fr as F
 
     dev = torch.device("cuda:0")
-    box, chunk = args.box, args.chunk
+    box, chunk = args.box, args.chunk or 1000
     n_total = max(args.slices, chunk * (args.steps + args.warmup)) if args.steps else args.slices
     s = F.FRSettings(F.FR_LOOKUP, 0, F.FR_PRECISE, 15.0, 1.9)
     n_chunks = n_total // chunk

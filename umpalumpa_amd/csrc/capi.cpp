@@ -179,6 +179,8 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
       desc->fft.x > (uint64_t)INT_MAX || desc->fft.y > (uint64_t)INT_MAX ||
       desc->fft.n > (uint64_t)INT_MAX || desc->blobTable > (uint64_t)INT_MAX)
     return UMPA_ERR_UNSUPPORTED;
+  // a slice is read through a buffer resource with a 32-bit byte range (complex float pixels)
+  if (desc->fft.x * desc->fft.y * 8 > (uint64_t)INT_MAX) return UMPA_ERR_UNSUPPORTED;
   try {
     op->desc = *desc;
     op->settings = *s;
@@ -232,7 +234,7 @@ static int fr_check_execute(umpa_fr_op *op, const void *fft, const float *scales
   if (!aligned(volume, 8) || !aligned(weights, 4) || !aligned(fft, half ? 4 : 8) ||
       !aligned(spaces, 4) || (table && !aligned(table, 4)))
     return UMPA_ERR_INVALID_ARG;
-  if (half && !scales) return UMPA_ERR_INVALID_ARG;
+  if (half && (!scales || !aligned(scales, 4))) return UMPA_ERR_INVALID_ARG;
   return UMPA_OK;
 }
 
@@ -427,6 +429,12 @@ int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint6
     e = upload_bricks(bricks, &items);
     if (e != hipSuccess) return from_hip(e);
     int *&dst = v.fastMath ? op->dBricksSplit : op->dBricks;
+    // executes already enqueued on the stream may still read the old schedule
+    e = hipStreamSynchronize(op->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(items);
+      return from_hip(e);
+    }
     (void)hipFree(dst);
     dst = items;
     (v.fastMath ? op->nBricksSplit : op->nBricks) = (int)bricks.size();
