@@ -49,7 +49,7 @@ def solo(job_host, strategy, chunk=64):
     return job.volume.cpu().numpy(), job.weights.cpu().numpy()
 
 
-@pytest.mark.parametrize("policy", ["srpt", "fair"])
+@pytest.mark.parametrize("policy", ["srpt", "aged", "oldest", "fair"])
 def test_exact_alone_equals_mix_and_oracle(policy):
     chunk = 64
     mix = make_jobs(chunk=chunk)

@@ -10,6 +10,7 @@ enqueued on torch's current stream of that device unless a stream is given.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 
 import numpy as np
 
@@ -34,6 +35,12 @@ def _stream_handle(stream):
     if stream is None:
         stream = torch.cuda.current_stream()
     return C.c_void_p(stream.cuda_stream)
+
+
+def _own(obj, destroy: str) -> None:
+    """Destroys obj._h with the given C entry point when obj is collected or the interpreter
+    exits (the entry point is bound now, so it outlives the module's globals at shutdown)."""
+    obj._finalizer = weakref.finalize(obj, getattr(_lib.load(), destroy), obj._h)
 
 
 def _ptr(t):
@@ -146,15 +153,10 @@ class FourierReconstruction:
         self._h = C.c_void_p()
         check(_lib.load().umpa_fr_create(C.byref(self._h), device, _stream_handle(self._stream)),
               "umpa_fr_create")
+        _own(self, "umpa_fr_destroy")
         self.desc = None
         self.settings = None
         self.last_status = 0
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            _lib.load().umpa_fr_destroy(h)
-            self._h = None
 
     def init(self, desc: FRDesc, settings: FRSettings) -> bool:
         self.last_status = _lib.load().umpa_fr_init(self._h, C.byref(desc), C.byref(settings))
@@ -252,13 +254,8 @@ class FRBatch:
         check(_lib.load().umpa_fr_batch_create(C.byref(self._h), device,
                                                C.c_void_p(self.stream.cuda_stream)),
               "umpa_fr_batch_create")
+        _own(self, "umpa_fr_batch_destroy")
         self._jobs = None
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            _lib.load().umpa_fr_batch_destroy(h)
-            self._h = None
 
     def _follow(self):
         import torch
@@ -342,13 +339,8 @@ class FourierTransform:
         self._h = C.c_void_p()
         check(_lib.load().umpa_fft_create(C.byref(self._h), device,
                                           _stream_handle(self._stream)), "umpa_fft_create")
+        _own(self, "umpa_fft_destroy")
         self.last_status = 0
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            _lib.load().umpa_fft_destroy(h)
-            self._h = None
 
     def init(self, spatial, n: int, direction: int = FFT_FORWARD,
              precision: int = FFT_F32) -> bool:
@@ -377,13 +369,8 @@ class FourierProcessing:
         self._h = C.c_void_p()
         check(_lib.load().umpa_fp_create(C.byref(self._h), device, _stream_handle(self._stream)),
               "umpa_fp_create")
+        _own(self, "umpa_fp_destroy")
         self.last_status = 0
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h:
-            _lib.load().umpa_fp_destroy(h)
-            self._h = None
 
     def init(self, desc: FPDesc, settings: FPSettings) -> bool:
         self.last_status = _lib.load().umpa_fp_init(self._h, C.byref(desc), C.byref(settings))

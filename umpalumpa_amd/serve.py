@@ -9,9 +9,10 @@ inserted by the same launches.  A job's voxels receive its spaces in index order
 is in flight, so its result is the same alone and in the mix (bit for bit under the exact
 strategy).  Ticks are enqueued ahead of the GPU (up to ``depth`` in flight) and completions
 are noticed through stream tickets, so the host loop never waits on an idle GPU.  By default
-each tick serves the admitted jobs with the least remaining work first, up to a work quantum
-that fills the GPU (shortest-remaining-work first over chunks), which keeps short jobs' latency
-near their isolated runtime when long jobs share the GPU.
+each tick serves the admitted jobs in order of remaining work (in seconds, at the measured
+insertion rate) minus twice the time they have waited, up to a work quantum that fills the GPU:
+short jobs overtake long ones, which keeps their latency near their isolated runtime, until a
+long job has waited long enough to take its turn, which bounds the worst latency.
 
 The reference has no serving layer; its harness runs one reconstruction at a time
 (examples/fourier_reconstruction/fr.cpp:9-76).
@@ -75,7 +76,8 @@ class Server:
 
     def __init__(self, device: int = 0, chunk: int = 512, hbm_budget: float = 200e9,
                  strategy: int = F.FR_FAST_MATH, depth: int = 3, alpha: float = 15.0,
-                 max_active: int = 64, policy: str = "srpt", tick_work: float = 4 * 512 * 257 ** 2):
+                 max_active: int = 64, policy: str = "aged", tick_work: float = 4 * 512 * 257 ** 2,
+                 age_weight: float = 2.0):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda", device)
@@ -87,11 +89,18 @@ class Server:
         self.max_active = max_active
         # "srpt": each tick takes the admitted jobs with the least remaining work first, until
         # the tick holds `tick_work` (enough to fill the GPU), so short jobs overtake long ones;
+        # "oldest": the same quantum filled in arrival order, which bounds the worst latency
+        # (the longest job is never overtaken, later jobs fill what it leaves of the GPU);
+        # "aged": the same quantum by remaining work in seconds (at the measured insertion
+        # rate) minus `age_weight` times the time the job has waited, so a long job that has
+        # waited is no longer overtaken by every newcomer;
         # "fair": every admitted job advances by one chunk per tick
-        if policy not in ("srpt", "fair"):
+        if policy not in ("srpt", "oldest", "aged", "fair"):
             raise ValueError(policy)
         self.policy = policy
         self.tick_work = tick_work
+        self.age_weight = age_weight
+        self._rate = None          # work units per second, measured from completed ticks
         self.batch = F.FRBatch(device)
         self._tables = {}
         self.used = 0
@@ -145,7 +154,8 @@ class Server:
         with admitted/done times (seconds from the start)."""
         pending = collections.deque(sorted(jobs, key=lambda j: j.arrival))
         active: list[Job] = []
-        inflight = collections.deque()   # tickets of enqueued ticks
+        inflight = collections.deque()   # (ticket, work) of enqueued ticks
+        busy_since, done_work = None, 0.0
         t0 = time.perf_counter()
         self.batch.set_count_updates(count_updates)
         self.total_updates = 0
@@ -163,8 +173,12 @@ class Server:
                     pending[0].footprint(self.chunk) > self.budget:
                 raise ValueError(f"job {pending[0].job_id} needs more than the HBM budget")
             # completions
-            while inflight and self.batch.query(inflight[0]):
-                inflight.popleft()
+            while inflight and self.batch.query(inflight[0][0]):
+                done_work += inflight.popleft()[1]
+                if busy_since is not None and now > busy_since:
+                    self._rate = done_work / (now - busy_since)
+            if not inflight:
+                busy_since, done_work = None, 0.0
             for j in list(active):
                 if j._next_chunk == len(j._chunk_spaces) and self.batch.query(j._last_ticket):
                     j.done = time.perf_counter() - t0
@@ -172,8 +186,14 @@ class Server:
                     self.used -= j.footprint(self.chunk)
                     finished.append(j)
             ready = [j for j in active if j._next_chunk < len(j._chunk_spaces)]
-            if ready and self.policy == "srpt":
-                ready.sort(key=lambda j: (j.remaining_cost(self.chunk), j.arrival))
+            if ready and self.policy in ("srpt", "oldest", "aged"):
+                if self.policy == "oldest":
+                    ready.sort(key=lambda j: (j.arrival, j.job_id))
+                elif self.policy == "aged" and self._rate:
+                    ready.sort(key=lambda j: (j.remaining_cost(self.chunk) / self._rate -
+                                              self.age_weight * (now - j.arrival), j.arrival))
+                else:
+                    ready.sort(key=lambda j: (j.remaining_cost(self.chunk), j.arrival))
                 picked, work = [], 0.0
                 for j in ready:
                     if picked and work >= self.tick_work:
@@ -191,7 +211,10 @@ class Server:
                 if count_updates:    # (the counting variant waits for each step)
                     self.total_updates += self.batch.last_updates()
                 ticket = self.batch.record()
-                inflight.append(ticket)
+                if busy_since is None:
+                    busy_since = time.perf_counter() - t0
+                inflight.append((ticket, sum(j.chunk_cost(j._next_chunk, self.chunk)
+                                             for j in ready)))
                 for j in ready:
                     j._next_chunk += 1
                     j._last_ticket = ticket
