@@ -141,6 +141,19 @@ def test_box256_bit_exact_subset():
     assert _bits_equal(v, ov) and _bits_equal(w, ow)
 
 
+def test_box1024_bit_exact_few_slices():
+    """The largest volume of the workloads (1025^3, 8.6 GB + 4.3 GB weights: voxel offsets
+    past 2^31 bytes) with a few slices, exact strategy vs the oracle, bit for bit."""
+    s = settings()
+    fft, spaces, desc, table = problem(1024, 4, s, seed=13)
+    ov, ow, _ = run_oracle(s, fft, spaces, table, 1024)
+    v, w, _ = run_hip(s, fft, spaces, desc, table, 1024)
+    assert np.array_equal(v.view(np.uint32), ov.view(np.uint32))
+    del v, ov
+    assert np.array_equal(w.view(np.uint32), ow.view(np.uint32))
+    assert w.max() > 0
+
+
 def test_box256_weights_do_not_depend_on_values():
     """Weights depend on geometry only; volume is linear in the slice values."""
     s = settings()

@@ -211,12 +211,10 @@ __device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buff
   return o;
 }
 
-// load_row for the exact FP32 kernels: a row whose 16-byte loads straddle the slice end is
-// read pixel by pixel.  This compiler merges those pixel loads back into the same 16-byte
-// loads, so both sides load alike; what the branch changes is scheduling: it makes each row its
-// own region, in which this compiler issues every row's loads and table reads ahead of the
-// sums.  Without it, the exact FP32 kernel waited on each table read in turn (2.70 instead of
-// 2.06 ms per 1,000-slice chunk at box 256), while the fast-math kernels ran faster without it.
+// load_row for the exact FP32 Kaiser-mode kernels: a row whose 16-byte loads straddle the slice
+// end is read pixel by pixel.  This compiler merges those pixel loads back into the same
+// 16-byte loads, so both sides load alike; the branch only splits the rows into scheduling
+// regions, the form these kernels were measured and kept in.
 template <bool kHalf, int kNB>
 __device__ __forceinline__ void load_row_regions(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff,
                                                  uint32_t sliceBytes, Pixel<kHalf> (&px)[kNB])
@@ -312,18 +310,41 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
     // outside the blob or the slice weigh exactly 0 (the table's zero entry), so with finite
     // slice values they add +-0, which changes no running sum (the sums start at +0 and are
     // never -0): the result equals the reference's skip-the-pixel loop bit for bit.
-    constexpr bool kRegions = !kHalf && !kFastMath;
+    // The exact FP32 lookup kernel issues each row's loads and table reads in a scheduling
+    // region of its own, all four rows before the sums: left to itself this compiler waited on
+    // every table read in turn there (2.70 ms per 1,000-slice chunk at box 256 instead of
+    // 1.95 ms).  The fast-math and FP16 kernels run faster with the compiler's own order
+    // (1.49 / 1.42 ms instead of 1.53 / 1.46 ms with the regions).
+    constexpr bool kRegions = !kHalf && !kFastMath && kMode == kModeLookup;
+    Pixel<kHalf> pr[kNB][kNB];
+    float wr[kNB][kNB];
+    if constexpr (kRegions) {
+#pragma unroll
+      for (int ii = 0; ii < kNB; ++ii) {
+        load_row<kHalf, kNB>(rsrc, base + ii * rowStride, pr[ii]);
+#pragma unroll
+        for (int jj = 0; jj < kNB; ++jj)
+          wr[ii][jj] = candidate_weight<kMode, kOrder, kFastMath, kTableLds>(
+            xd2[jj] + yz[ii], r2, nTable, tbl, c);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
 #pragma unroll
     for (int ii = 0; ii < kNB; ++ii) {
       Pixel<kHalf> px[kNB];
-      if constexpr (kRegions)
+      if constexpr (kRegions) {
+#pragma unroll
+        for (int jj = 0; jj < kNB; ++jj) px[jj] = pr[ii][jj];
+      } else if constexpr (!kHalf && !kFastMath) {
         load_row_regions<kHalf, kNB>(rsrc, base + ii * rowStride, sliceBytes, px);
-      else
+      } else {
         load_row<kHalf, kNB>(rsrc, base + ii * rowStride, px);
+      }
 #pragma unroll
       for (int jj = 0; jj < kNB; ++jj) {
-        const float wb = candidate_weight<kMode, kOrder, kFastMath, kTableLds>(
-          xd2[jj] + yz[ii], r2, nTable, tbl, c);
+        const float wb = kRegions ? wr[ii][jj]
+                                  : candidate_weight<kMode, kOrder, kFastMath, kTableLds>(
+                                      xd2[jj] + yz[ii], r2, nTable, tbl, c);
         const float weight = kUnitWeight ? wb : wb * dw;
         w += weight;
         if constexpr (kFastMath) {
