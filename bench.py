@@ -34,9 +34,11 @@ reports the capacity throughput and, per policy and rate, p50/p99 job latency an
 over the p99 of the jobs' isolated runtimes (median and worst over the sequences).
 
 --workload pipeline: the harness's chunk loop from particle images (examples/fourier_reconstruction/
-fr.cpp:46-62): 1,000 float images 256 x 256 per step -> batched R2C FFT (hipFFT) -> crop to
-128 x 256 with centre / normalise / shift -> insertion into 257^3, one stream, traverse spaces
-made beforehand.  Reports GVoxel-updates/s, images/s and the time of each stage.
+fr.cpp:46-62): 1,000 float images 256 x 256 per step -> R2C FFT and crop to 128 x 256 with
+centre / normalise / shift, fused (hipFFT rows, one kernel for the columns and the crop) ->
+insertion into 257^3, one stream, traverse spaces made beforehand.  Reports GVoxel-updates/s,
+images/s and the time of each stage, and the same steps with the two separate operations
+(hipFFT 2-D, then the crop kernel).
 
 --workload large_d7: one reconstruction at box 1024 (513 x 1024 FP16 slices, D7 symmetry:
 14 operators per slice, FP64 on the host) into a 1025^3 volume, slices streamed through HBM
@@ -481,21 +483,34 @@ def run_pipeline(args):
         pipe.insert(pool[k % 4], spaces=spaces[k])
         updates += pipe.op.last_updates()
     pipe.op.set_count_updates(False)
-    for k in range(args.warmup):
-        pipe.insert(pool[k % 4], spaces=spaces[args.steps + k])
-    torch.cuda.synchronize()
-    pipe.volume.zero_()
-    pipe.weights.zero_()
-    torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    for k in range(args.steps):
-        pipe.insert(pool[k % 4], spaces=spaces[k], events=evs[k])
-    torch.cuda.synchronize()
-    stage = np.array([[e[i].elapsed_time(e[i + 1]) for i in range(3)] for e in evs])  # ms
-    total_ms = evs[0][0].elapsed_time(evs[-1][3])
+
+    def timed(p):
+        for k in range(args.warmup):
+            p.insert(pool[k % 4], spaces=spaces[args.steps + k])
+        torch.cuda.synchronize()
+        p.volume.zero_()
+        p.weights.zero_()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)]
+               for _ in range(args.steps)]
+        for k in range(args.steps):
+            p.insert(pool[k % 4], spaces=spaces[k], events=evs[k])
+        torch.cuda.synchronize()
+        stage = np.array([[e[i].elapsed_time(e[i + 1]) for i in range(3)] for e in evs])  # ms
+        return stage, evs[0][0].elapsed_time(evs[-1][3])
+
+    stage, total_ms = timed(pipe)
     hx = box // 2 + 1
-    fft_bytes = chunk * (box * box * 4 + hx * box * 8)
-    crop_bytes = chunk * (hx * box * 8 + (box // 2) * box * 8)
+    img_bytes, out_bytes = box * box * 4, (box // 2) * box * 8
+    fused = {"fft_crop_ms": float(stage[:, 0].mean() + stage[:, 1].mean()),
+             "insert_ms": float(stage[:, 2].mean()),
+             # row spectra written and read once, images in, crop out
+             "fft_crop_hbm_gbs": chunk * (img_bytes + 2 * hx * box * 8 + out_bytes)
+             / ((stage[:, 0].mean() + stage[:, 1].mean()) * 1e-3) / 1e9}
+    sep = ImagePipeline(box, chunk, s, strategy=strategy, fused=False)
+    stage2, total2 = timed(sep)
+    separate = {"ms_per_step": total2 / args.steps, "fft_ms": float(stage2[:, 0].mean()),
+                "crop_ms": float(stage2[:, 1].mean()), "insert_ms": float(stage2[:, 2].mean())}
     result = {
         "metric": "GVoxel-updates/s + HBM GB/s (box 256); p50/p99 job latency, batched vs plain",
         "value": updates / (total_ms * 1e-3) / 1e9, "unit": "GVoxel-updates/s",
@@ -507,13 +522,12 @@ def run_pipeline(args):
                                f"{box}x{box} -> R2C FFT -> crop {box // 2}x{box} "
                                f"(centre, normalise, shift) -> insert into {box + 1}^3, "
                                f"chunks of {chunk}",
+                   "fft_crop": "fused" if pipe.fused else "separate",
                    "strategy": args.strategy, "parallelism": "single GPU"},
         "voxel_updates": int(updates),
         "images_per_s": args.steps * chunk / (total_ms * 1e-3),
-        "stage_ms": {"fft": float(stage[:, 0].mean()), "crop": float(stage[:, 1].mean()),
-                     "insert": float(stage[:, 2].mean())},
-        "stage_hbm_gbs": {"fft": fft_bytes / (stage[:, 0].mean() * 1e-3) / 1e9,
-                          "crop": crop_bytes / (stage[:, 1].mean() * 1e-3) / 1e9},
+        "stage": fused if pipe.fused else separate,
+        "separate_operations": separate,
     }
     print(json.dumps(result))
 

@@ -339,6 +339,28 @@ int umpa_fft_synchronize(umpa_fft_op *op);
 int umpa_fft_cleanup(umpa_fft_op *op);
 void umpa_fft_destroy(umpa_fft_op *op);
 
+/* ---- fused FFT + FP: the harness's ConvertToFFT + Crop (fr.cpp:134-203) in one operation --
+ * No single counterpart in the reference: it replaces the pair FFTCUDA (fft_cuda.cpp:51-69)
+ * then FPCUDA (fp_cuda.cpp:144-198) for 2-D images, with the FP semantics of umpa_fp_*
+ * (crop, filter, normalise, centre, max frequency, shift) applied to the forward R2C
+ * transform of the images.  Rows are transformed by hipFFT into a workspace; the column
+ * transforms and the FP step run in one gfx950 kernel.  Results equal the two operations in
+ * sequence up to the rounding of the transform. */
+typedef struct umpa_fft_fp_op umpa_fft_fp_op;
+
+int umpa_fft_fp_create(umpa_fft_fp_op **op, int device, void *stream);
+/* desc as for umpa_fp_init, with in = the R2C size of inSpatial (in.x = inSpatial.x / 2 + 1,
+ * in.y = inSpatial.y), 2-D images, inSpatial.y a power of two from 4 to 1024 (else
+ * UMPA_ERR_UNSUPPORTED); allocates the workspace for desc->inSpatial.n images */
+int umpa_fft_fp_init(umpa_fft_fp_op *op, const umpa_fp_desc *desc, const umpa_fp_settings *s);
+/* images: float [count][inSpatial.y][inSpatial.x] device memory; out: complex float
+ * [count][out.y][out.x]; filter float [out.y][out.x] (NULL unless applyFilter) */
+int umpa_fft_fp_execute(umpa_fft_fp_op *op, const float *images, uint64_t count, void *out,
+                        const float *filter);
+int umpa_fft_fp_synchronize(umpa_fft_fp_op *op);
+int umpa_fft_fp_cleanup(umpa_fft_fp_op *op);
+void umpa_fft_fp_destroy(umpa_fft_fp_op *op);
+
 /* ---- misc --------------------------------------------------------------------------------- */
 const char *umpa_status_string(int status);
 /* library version, for the loader's sanity check */

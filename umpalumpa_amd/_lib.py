@@ -155,6 +155,12 @@ PROTOTYPES = [
     ("umpa_fp_synchronize", C.c_int, [_P]),
     ("umpa_fp_cleanup", C.c_int, [_P]),
     ("umpa_fp_destroy", None, [_P]),
+    ("umpa_fft_fp_create", C.c_int, [C.POINTER(_P), C.c_int, _P]),
+    ("umpa_fft_fp_init", C.c_int, [_P, C.POINTER(FPDesc), C.POINTER(FPSettings)]),
+    ("umpa_fft_fp_execute", C.c_int, [_P, _P, _U64, _P, _P]),
+    ("umpa_fft_fp_synchronize", C.c_int, [_P]),
+    ("umpa_fft_fp_cleanup", C.c_int, [_P]),
+    ("umpa_fft_fp_destroy", None, [_P]),
 ]
 
 _lib = None

@@ -8,9 +8,13 @@
 #include <hipfft/hipfft.h>
 
 #include <climits>
+#include <cmath>
+#include <complex>
 #include <map>
 #include <new>
+#include <vector>
 
+#include "launch.hpp"
 #include "umpalumpa_amd.h"
 
 namespace {
@@ -29,6 +33,11 @@ int from_fft(hipfftResult r)
   if (r == HIPFFT_INVALID_SIZE || r == HIPFFT_INVALID_VALUE) return UMPA_ERR_INVALID_ARG;
   return UMPA_ERR_DEVICE;
 }
+
+// Size::IsValid (data/size.hpp:43-57) for the 2-D images of the fused operation
+bool valid_2d(const umpa_size &s) { return s.n >= 1 && s.z == 1 && s.y >= 2 && s.x >= 2; }
+
+bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
 struct DeviceGuard {
   int prev = -1;
@@ -95,6 +104,55 @@ struct umpa_fft_op {
       size_t work = 0;
       r = hipfftMakePlanMany(h, rank, n, nullptr, 1, fwd ? spatial : freq, nullptr, 1,
                              fwd ? freq : spatial, type, (int)count, &work);
+    }
+    if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, stream);
+    if (r != HIPFFT_SUCCESS) {
+      if (h) (void)hipfftDestroy(h);
+      return from_fft(r);
+    }
+    plans[count] = h;
+    out = h;
+    return UMPA_OK;
+  }
+};
+
+struct umpa_fft_fp_op {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool initialized = false;
+  umpa_fp_desc desc{};
+  umpa_fp_settings settings{};
+  std::map<uint64_t, hipfftHandle> plans;  // row transforms, by image count
+  void *work = nullptr;                    // row spectra [n][y][x/2+1] complex float
+  void *twiddles = nullptr;                // exp(-2 pi i k / y), k < y, complex float
+
+  void release()
+  {
+    for (auto &p : plans) (void)hipfftDestroy(p.second);
+    plans.clear();
+    if (work) (void)hipFree(work);
+    if (twiddles) (void)hipFree(twiddles);
+    work = nullptr;
+    twiddles = nullptr;
+    initialized = false;
+  }
+
+  // batched 1-D R2C over every image row: count * y rows of x reals
+  int plan(uint64_t count, hipfftHandle &out)
+  {
+    auto it = plans.find(count);
+    if (it != plans.end()) {
+      out = it->second;
+      return UMPA_OK;
+    }
+    const umpa_size &s = desc.inSpatial;
+    int n[1] = { (int)s.x };
+    hipfftHandle h = nullptr;
+    hipfftResult r = hipfftCreate(&h);
+    if (r == HIPFFT_SUCCESS) {
+      size_t ws = 0;
+      r = hipfftMakePlanMany(h, 1, n, nullptr, 1, (int)s.x, nullptr, 1, (int)(s.x / 2 + 1),
+                             HIPFFT_R2C, (int)(count * s.y), &ws);
     }
     if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, stream);
     if (r != HIPFFT_SUCCESS) {
@@ -197,6 +255,141 @@ void umpa_fft_destroy(umpa_fft_op *op)
     (void)hipStreamSynchronize(op->stream);
     op->release();
   }
+  delete op;
+}
+
+// ---- fused FFT + FP ---------------------------------------------------------------------
+int umpa_fft_fp_create(umpa_fft_fp_op **op, int device, void *stream)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  *op = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+    return UMPA_ERR_DEVICE;
+  auto *h = new (std::nothrow) umpa_fft_fp_op();
+  if (!h) return UMPA_ERR_OUT_OF_MEMORY;
+  h->device = device;
+  h->stream = static_cast<hipStream_t>(stream);
+  *op = h;
+  return UMPA_OK;
+}
+
+int umpa_fft_fp_init(umpa_fft_fp_op *op, const umpa_fp_desc *desc, const umpa_fp_settings *s)
+{
+  if (!op || !desc || !s) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  op->release();
+  const umpa_size &sp = desc->inSpatial;
+  // the FP checks of umpa_fp_init (afp.hpp:45-52, settings.hpp:38-43) on 2-D payloads
+  if (!valid_2d(sp) || !valid_2d(desc->in) || !valid_2d(desc->out)) return UMPA_ERR_INVALID_ARG;
+  if (desc->in.x != sp.x / 2 + 1 || desc->in.y != sp.y || desc->in.n != sp.n)
+    return UMPA_ERR_INVALID_ARG;
+  if (desc->out.n < desc->in.n) return UMPA_ERR_INVALID_ARG;
+  if (desc->out.x > desc->in.x || desc->out.y > desc->in.y) return UMPA_ERR_UNSUPPORTED;
+  if (s->hasMaxFreq && !(s->maxFreq >= 0.f && s->maxFreq <= 0.5f)) return UMPA_ERR_INVALID_ARG;
+  if (desc->inPaddedSpatialSingle == 0) return UMPA_ERR_INVALID_ARG;
+  // column transforms: power-of-two heights that fit the kernel's LDS tiles
+  if (sp.y < 4 || sp.y > 1024 || (sp.y & (sp.y - 1)) != 0) return UMPA_ERR_UNSUPPORTED;
+  if (sp.x > (uint64_t)INT_MAX || sp.n * sp.y > (uint64_t)INT_MAX ||
+      sp.n * sp.y * sp.x > (uint64_t)INT_MAX)
+    return UMPA_ERR_UNSUPPORTED;
+  op->desc = *desc;
+  op->settings = *s;
+  try {
+    std::vector<std::complex<float>> tw(sp.y);
+    const double pi = 3.14159265358979323846;
+    for (uint64_t k = 0; k < sp.y; ++k) {
+      const double a = -2.0 * pi * (double)k / (double)sp.y;
+      tw[k] = std::complex<float>((float)std::cos(a), (float)std::sin(a));
+    }
+    hipError_t e = hipMalloc(&op->twiddles, tw.size() * sizeof(tw[0]));
+    if (e == hipSuccess)
+      e = hipMemcpy(op->twiddles, tw.data(), tw.size() * sizeof(tw[0]), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMalloc(&op->work, sp.n * sp.y * (sp.x / 2 + 1) * 8);
+    if (e == hipSuccess) e = umpa::prepare_fft_fp_columns((int)sp.y);
+    if (e != hipSuccess) {
+      op->release();
+      return from_hip(e);
+    }
+  } catch (const std::bad_alloc &) {
+    op->release();
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+  hipfftHandle h = nullptr;
+  const int st = op->plan(sp.n, h);
+  if (st != UMPA_OK) {
+    op->release();
+    return st;
+  }
+  op->initialized = true;
+  return UMPA_OK;
+}
+
+int umpa_fft_fp_execute(umpa_fft_fp_op *op, const float *images, uint64_t count, void *out,
+                        const float *filter)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  if (!op->initialized) return UMPA_ERR_NOT_INITIALIZED;
+  if (count > op->desc.inSpatial.n) return UMPA_ERR_NOT_EQUIVALENT;
+  if (!images || !out || count == 0) return UMPA_ERR_REJECTED;
+  if (op->settings.applyFilter && !filter) return UMPA_ERR_INVALID_ARG;
+  if (!aligned(images, 4) || !aligned(out, 8) || (filter && !aligned(filter, 4)))
+    return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  hipfftHandle h = nullptr;
+  int st = op->plan(count, h);
+  if (st != UMPA_OK) return st;
+  hipfftResult r = hipfftExecR2C(h, const_cast<hipfftReal *>(images),
+                                 static_cast<hipfftComplex *>(op->work));
+  if (r != HIPFFT_SUCCESS) return from_fft(r);
+  const umpa_fp_desc &d = op->desc;
+  umpa::FftFpArgs a{};
+  a.work = op->work;
+  a.workX = d.inSpatial.x / 2 + 1;
+  a.y = (int)d.inSpatial.y;
+  a.inY = d.in.y;
+  a.out = out;
+  a.filter = filter;
+  a.twiddles = op->twiddles;
+  a.spatialX = d.inSpatial.x;
+  a.spatialY = d.inSpatial.y;
+  a.outX = d.out.x;
+  a.outY = d.out.y;
+  a.outSingle = d.out.x * d.out.y;
+  a.n = count;
+  a.normFactor = 1.f / static_cast<float>(d.inPaddedSpatialSingle);
+  a.maxFreqSquare = op->settings.hasMaxFreq ? op->settings.maxFreq : 0.f;
+  a.applyFilter = op->settings.applyFilter != 0;
+  a.normalize = op->settings.normalize != 0;
+  a.center = op->settings.center != 0;
+  a.cropFreq = op->settings.hasMaxFreq != 0;
+  a.shift = op->settings.shift != 0;
+  return from_hip(umpa::launch_fft_fp_columns(a, op->stream));
+}
+
+int umpa_fft_fp_synchronize(umpa_fft_fp_op *op)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  return from_hip(hipStreamSynchronize(op->stream));
+}
+
+int umpa_fft_fp_cleanup(umpa_fft_fp_op *op)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  DeviceGuard g(op->device);
+  if (op->initialized) (void)hipStreamSynchronize(op->stream);
+  op->release();
+  return UMPA_OK;
+}
+
+void umpa_fft_fp_destroy(umpa_fft_fp_op *op)
+{
+  if (!op) return;
+  umpa_fft_fp_cleanup(op);
   delete op;
 }
 

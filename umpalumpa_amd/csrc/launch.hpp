@@ -85,4 +85,25 @@ struct FpArgs {
 
 hipError_t launch_fp_scale(const FpArgs &a, int numCUs, hipStream_t st);
 
+// Column pass of the fused FFT + FP step: `work` holds the row transforms [n][y][workX]
+// (complex float, workX = spatial x / 2 + 1); y is a power of two from 4 to 1024.
+struct FftFpArgs {
+  const void *work;
+  uint64_t workX;
+  int y;                         // rows (= frequency rows of the input spectrum)
+  uint64_t inY;
+  void *out;                     // complex float [n][outY][outX]
+  const float *filter;
+  const void *twiddles;         // complex float exp(-2 pi i k / y), k < y
+  uint64_t spatialX, spatialY;
+  uint64_t outX, outY, outSingle;
+  uint64_t n;
+  float normFactor;
+  float maxFreqSquare;
+  int applyFilter, normalize, center, cropFreq, shift;
+};
+
+hipError_t prepare_fft_fp_columns(int y);
+hipError_t launch_fft_fp_columns(const FftFpArgs &a, hipStream_t st);
+
 }  // namespace umpa

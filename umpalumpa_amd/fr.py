@@ -26,7 +26,7 @@ __all__ = [
     "FourierReconstruction", "FourierProcessing", "create_constants", "fill_blob_table",
     "compute_traverse_space", "traverse_spaces", "generate_traverse_spaces", "BatchJob",
     "FRBatch", "SchedParams", "symmetry_operators", "finalize", "FourierTransform",
-    "FFT_FORWARD", "FFT_INVERSE", "FFT_F32", "FFT_F64",
+    "FFT_FORWARD", "FFT_INVERSE", "FFT_F32", "FFT_F64", "FourierTransformProcessing",
 ]
 
 
@@ -356,6 +356,39 @@ class FourierTransform:
 
     def synchronize(self) -> None:
         check(_lib.load().umpa_fft_synchronize(self._h), "synchronize")
+
+
+# ---- fused FFT + FP --------------------------------------------------------------------------
+class FourierTransformProcessing:
+    """The forward R2C transform of 2-D images followed by the FP step (crop / filter /
+    normalise / centre / shift), fused (umpa_fft_fp_*): hipFFT for the rows, one gfx950
+    kernel for the column transforms and the FP arithmetic.  Same descriptor and settings as
+    FourierProcessing, with desc.in the R2C size of desc.inSpatial; image heights are powers of
+    two from 4 to 1024."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self.device = device
+        self._stream = stream if stream is not None else torch.cuda.current_stream(device)
+        self._h = C.c_void_p()
+        check(_lib.load().umpa_fft_fp_create(C.byref(self._h), device,
+                                             _stream_handle(self._stream)), "umpa_fft_fp_create")
+        _own(self, "umpa_fft_fp_destroy")
+        self.last_status = 0
+
+    def init(self, desc: FPDesc, settings: FPSettings) -> bool:
+        self.last_status = _lib.load().umpa_fft_fp_init(self._h, C.byref(desc),
+                                                        C.byref(settings))
+        return self.last_status == 0
+
+    def execute(self, images, count: int, out, filt=None) -> bool:
+        """images: float32 [count, y, x]; out: complex64 [count, out.y, out.x]."""
+        self.last_status = _lib.load().umpa_fft_fp_execute(self._h, _ptr(images), count,
+                                                           _ptr(out), _ptr(filt))
+        return self.last_status == 0
+
+    def synchronize(self) -> None:
+        check(_lib.load().umpa_fft_fp_synchronize(self._h), "synchronize")
 
 
 # ---- AFP ----------------------------------------------------------------------------------
