@@ -87,6 +87,8 @@ int check_job(const umpa_fr_job &j, int strategy, JobInfo &out)
       d.fft.y > (uint64_t)INT_MAX || d.fft.n > (uint64_t)INT_MAX ||
       d.traverseSpaces > (uint64_t)INT_MAX || d.blobTable > (uint64_t)INT_MAX)
     return UMPA_ERR_UNSUPPORTED;
+  // a slice is read through a buffer resource with a 32-bit byte range
+  if (d.fft.x * d.fft.y * 8 > (uint64_t)INT_MAX) return UMPA_ERR_UNSUPPORTED;
   const bool half = j.scales != nullptr;
   if (!j.fft || !j.volume || !j.weights) return UMPA_ERR_REJECTED;
   if (d.traverseSpaces > 0 && !j.spaces) return UMPA_ERR_INVALID_ARG;

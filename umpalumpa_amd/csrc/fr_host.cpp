@@ -281,7 +281,7 @@ std::vector<BrickOrigin> make_schedule(const std::vector<JobWork> &jobs, bool sp
       int parts = 1;
       const double at = fair * (double)sp.splitAt;
       if (splitHeavy && sp.maxParts > 1 && est > at && est > 4 * mean)
-        parts = std::min(sp.maxParts, (int)std::ceil(est / at));
+        parts = (int)std::min<double>(sp.maxParts, std::ceil(est / at));
       for (int k = 0; k < parts; ++k) {
         BrickOrigin b{ c.x, c.y, c.z | (j.job << 16), flags | (parts << 8) | (k << 16), 0.f };
         b.est = (float)(est / parts);
@@ -377,7 +377,11 @@ bool tuning_cache_lookup(const std::string &key, SchedParams &out)
     SchedParams p;
     int prio = 1;
     double ms = 0;
-    if (is >> k >> p.depthScale >> prio >> p.maxParts >> p.splitAt >> ms && k == key) {
+    // entries outside the knobs' valid ranges (a damaged or hand-edited file) are ignored
+    if (is >> k >> p.depthScale >> prio >> p.maxParts >> p.splitAt >> ms && k == key &&
+        std::isfinite(p.depthScale) && p.depthScale > 0.f && p.depthScale <= 64.f &&
+        p.maxParts >= 1 && p.maxParts <= 255 && std::isfinite(p.splitAt) && p.splitAt > 0.f &&
+        p.splitAt <= 1024.f) {
       p.priority = prio != 0;
       out = p;
       found = true;
