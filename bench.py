@@ -127,17 +127,23 @@ def uniform_rotations(n, seed):
         2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], axis=1).reshape(n, 3, 3)
 
 
-def measured_traffic(strategy):
-    """HBM bytes per launch of the box-256 insertion from the committed rocprofv3 summary
-    (FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings), or None."""
+def profile_figure(key, strategy):
+    """A per-launch figure of the box-256 insertion from the committed rocprofv3 summary
+    (`traffic_bytes_per_launch`: FETCH_SIZE and WRITE_SIZE scaled by a plain copy's readings;
+    `valu_insts_per_launch`: SQ_INSTS_VALU), or None."""
     path = os.path.join(ROOT, "profiles", "fr_insert_box256_pmc.md")
     try:
         for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return json.loads(line.split(":", 1)[1]).get(strategy)
     except OSError:
         pass
     return None
+
+
+# VALU issue peak: one wave64 vector instruction per cycle per CU (four 16-lane SIMDs, four
+# cycles each), 256 CUs at the 2.4 GHz peak engine clock
+VALU_PEAK_WAVE_INSTS = 256 * 2.4e9
 
 
 def cpu_baseline(box, fft_chunk, spaces_chunk, table):
@@ -750,10 +756,18 @@ def main():
         "other_fp32_strategy": other,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(args.strategy)
+                     "traffic": profile_figure("traffic_bytes_per_launch", args.strategy)
                      if (box, chunk) == (256, 1000) else None,
                      "kernel": "fr_insert_kernel", "alg_bytes_per_launch": b_alg},
     }
+    valu = profile_figure("valu_insts_per_launch", args.strategy) \
+        if (box, chunk) == (256, 1000) else None
+    if valu:
+        # the bound the kernel actually meets: vector-instruction issue
+        rate = valu / (per.mean() * 1e-3)
+        result["valu_issue"] = {"achieved": rate, "peak": VALU_PEAK_WAVE_INSTS,
+                                "unit": "wave64 instructions/s", "frac": rate / VALU_PEAK_WAVE_INSTS,
+                                "insts_per_launch": valu}
     if not args.no_cpu:
         nc = min(args.cpu_slices, chunk)
         sp0 = spaces[0][:nc]
