@@ -14,17 +14,15 @@
 #include <new>
 #include <vector>
 
+#include "hip_util.hpp"
 #include "launch.hpp"
 #include "umpalumpa_amd.h"
 
 namespace {
 
-int from_hip(hipError_t e)
-{
-  if (e == hipSuccess) return UMPA_OK;
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return UMPA_ERR_OUT_OF_MEMORY;
-  return UMPA_ERR_DEVICE;
-}
+using umpa::detail::aligned;
+using umpa::detail::DeviceGuard;
+using umpa::detail::from_hip;
 
 int from_fft(hipfftResult r)
 {
@@ -36,23 +34,6 @@ int from_fft(hipfftResult r)
 
 // Size::IsValid (data/size.hpp:43-57) for the 2-D images of the fused operation
 bool valid_2d(const umpa_size &s) { return s.n >= 1 && s.z == 1 && s.y >= 2 && s.x >= 2; }
-
-bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
-
-struct DeviceGuard {
-  int prev = -1;
-  bool ok = false;
-  explicit DeviceGuard(int device)
-  {
-    if (hipGetDevice(&prev) != hipSuccess) return;
-    ok = (prev == device) || hipSetDevice(device) == hipSuccess;
-  }
-  ~DeviceGuard()
-  {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
 
 }  // namespace
 

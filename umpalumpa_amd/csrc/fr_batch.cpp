@@ -25,43 +25,16 @@
 #include <vector>
 
 #include "fr_host.hpp"
+#include "hip_util.hpp"
 #include "launch.hpp"
 #include "umpalumpa_amd.h"
 
 namespace {
 
-int from_hip(hipError_t e)
-{
-  if (e == hipSuccess) return UMPA_OK;
-  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return UMPA_ERR_OUT_OF_MEMORY;
-  return UMPA_ERR_DEVICE;
-}
-
-struct DeviceGuard {
-  int prev = -1;
-  bool ok = false;
-  explicit DeviceGuard(int device)
-  {
-    if (hipGetDevice(&prev) != hipSuccess) return;
-    ok = (prev == device) || hipSetDevice(device) == hipSuccess;
-  }
-  ~DeviceGuard()
-  {
-    int cur = -1;
-    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
-  }
-};
-
-bool size_valid(const umpa_size &s)
-{
-  if (s.n < 1) return false;
-  const bool d3 = s.z >= 2 && s.y >= 2 && s.x >= 2;
-  const bool d2 = s.z == 1 && s.y >= 2 && s.x >= 2;
-  const bool d1 = s.z == 1 && s.y == 1 && s.x >= 1;
-  return d3 || d2 || d1;
-}
-
-bool aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
+using umpa::detail::aligned;
+using umpa::detail::DeviceGuard;
+using umpa::detail::from_hip;
+using umpa::detail::size_valid;
 
 // A validated job: its kernel variant and the record fields that do not depend on the step.
 struct JobInfo {
