@@ -310,18 +310,16 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
     // outside the blob or the slice weigh exactly 0 (the table's zero entry), so with finite
     // slice values they add +-0, which changes no running sum (the sums start at +0 and are
     // never -0): the result equals the reference's skip-the-pixel loop bit for bit.
-    // The exact FP32 lookup kernel issues each row's loads and table reads in a scheduling
-    // region of its own, all four rows before the sums: left to itself this compiler waited on
-    // every table read in turn there (2.70 ms per 1,000-slice chunk at box 256 instead of
-    // 1.95 ms).  The fast-math and FP16 kernels run faster with the compiler's own order
-    // (1.49 / 1.42 ms instead of 1.53 / 1.46 ms with the regions).
+    // The exact FP32 lookup kernel issues each row's table reads in a scheduling region of
+    // its own, all four rows before the sums: left to itself this compiler waited on every
+    // table read in turn there (2.70 ms per 1,000-slice chunk at box 256 instead of 1.94 ms).
+    // The fast-math and FP16 kernels run faster with the compiler's own order (1.49 / 1.42 ms
+    // instead of 1.53 / 1.46 ms with such regions).
     constexpr bool kRegions = !kHalf && !kFastMath && kMode == kModeLookup;
-    Pixel<kHalf> pr[kNB][kNB];
     float wr[kNB][kNB];
     if constexpr (kRegions) {
 #pragma unroll
       for (int ii = 0; ii < kNB; ++ii) {
-        load_row<kHalf, kNB>(rsrc, base + ii * rowStride, pr[ii]);
 #pragma unroll
         for (int jj = 0; jj < kNB; ++jj)
           wr[ii][jj] = candidate_weight<kMode, kOrder, kFastMath, kTableLds>(
@@ -333,8 +331,7 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
     for (int ii = 0; ii < kNB; ++ii) {
       Pixel<kHalf> px[kNB];
       if constexpr (kRegions) {
-#pragma unroll
-        for (int jj = 0; jj < kNB; ++jj) px[jj] = pr[ii][jj];
+        load_row<kHalf, kNB>(rsrc, base + ii * rowStride, px);
       } else if constexpr (!kHalf && !kFastMath) {
         load_row_regions<kHalf, kNB>(rsrc, base + ii * rowStride, sliceBytes, px);
       } else {
