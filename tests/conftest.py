@@ -19,8 +19,9 @@ def _ensure_built():
     """Tests load what is built and build only when nothing is."""
     lib = os.path.join(ROOT, "umpalumpa_amd", "libumpalumpa_amd.so")
     orc = os.path.join(ROOT, "oracle", "liboracle.so")
-    cpp = os.path.join(ROOT, "tests", "cpp", "_build", "fr_fp_tests")
-    if not (os.path.exists(lib) and os.path.exists(orc) and os.path.exists(cpp)):
+    cpp = [os.path.join(ROOT, "tests", "cpp", "_build", name)
+           for name in ("fr_fp_tests", "fr_harness")]
+    if not (os.path.exists(lib) and os.path.exists(orc) and all(map(os.path.exists, cpp))):
         import __graft_entry__
         __graft_entry__.build()
 
