@@ -239,9 +239,16 @@ __device__ __forceinline__ float candidate_weight(float d2, float r2, int nTable
                                                   const umpa_fr_constants &c)
 {
   if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
-    int aux = (int)__builtin_fmaf(d2, c.cIDeltaSqrt, 0.5f);
-    aux = aux < nTable ? aux : nTable;
-    return tbl[aux];
+    // 2^23 + d2 * cIDeltaSqrt rounds to an integer (float spacing 1 in [2^23, 2^24)), so the
+    // position, rounded to the nearest entry, sits in the low mantissa bits: no conversion
+    // instruction.  Clamped into the zero entry nTable (< 2^23), +inf included.
+    // The table's LDS address minus 4 * 0x4B000000 (the bits of 2^23) is one wave-uniform
+    // register, so a candidate's address is a single shift-and-add of the position's bits.
+    const float pos = fminf(__builtin_fmaf(d2, c.cIDeltaSqrt, 8388608.f), 8388608.f + (float)nTable);
+    typedef __attribute__((address_space(3))) const float lds_float;
+    const uint32_t biased = __builtin_amdgcn_readfirstlane(
+      (uint32_t)(uintptr_t)(lds_float *)tbl - (0x4B000000u << 2));
+    return *(lds_float *)(uintptr_t)(biased + (__float_as_uint(pos) << 2));
   } else {
     const bool in = d2 <= r2;
     if constexpr (kMode == kModeLookup) {
