@@ -117,3 +117,33 @@ def test_fft_errors():
     assert not op.execute(sp, fr, 3)                       # more images than at Init
     assert not op.execute(sp, sp, 2)                       # in place
     assert op.last_status == F._lib.UMPA_ERR_UNSUPPORTED
+
+
+def test_plans_for_many_counts_stay_bounded_and_correct():
+    """An Execute over each of 1 .. 13 images (more distinct counts than the eight plans an
+    operation keeps), then the Init count again: every result matches numpy."""
+    shape, n = (24, 16, 1), 13
+    rng = np.random.default_rng(7)
+    imgs = rng.standard_normal((n, 16, 24)).astype(np.float32)
+    d_img = torch.from_numpy(imgs).cuda()
+    want = np.fft.rfft2(imgs.astype(np.float64))
+    op = F.FourierTransform(0)
+    assert op.init(shape, n, F.FFT_FORWARD)
+    for m in list(range(1, n + 1)) + [n, 3]:
+        out = complex_tensor(shape, n)
+        assert op.execute(d_img, out, m)
+        op.synchronize()
+        got = out.cpu().numpy()
+        assert not got[m:].any()
+        err = np.linalg.norm(got[:m] - want[:m]) / np.linalg.norm(want[:m])
+        assert err <= 1e-6, (m, err)
+    fused = F.FourierTransformProcessing(0)
+    st = F.FPSettings(0, 0, 0, 0, 0, 0.0)
+    assert fused.init(F.fp_desc((24, 16), (24, 16), n), st)
+    for m in list(range(1, n + 1)) + [n]:
+        out = torch.zeros((n, 16, 13), dtype=torch.complex64, device="cuda")
+        assert fused.execute(d_img, m, out)
+        fused.synchronize()
+        got = out.cpu().numpy()
+        err = np.linalg.norm(got[:m] - want[:m]) / np.linalg.norm(want[:m])
+        assert err <= 1e-6, (m, err)

@@ -64,3 +64,21 @@ def test_fp_errors():
     assert op.last_status == _lib.UMPA_ERR_NOT_INITIALIZED
     st = F.FPSettings(0, 0, 0, 0, 1, 0.7)   # maxFreq above 1/2 (settings.hpp:38-43)
     assert not op.init(F.fp_desc((10, 10), (10, 10), 1), st)
+
+
+def test_fp_python_mirror_refuses_short_outputs():
+    """An output or filter shorter than the descriptor says is refused before the C ABI."""
+    import torch
+    op = F.FourierProcessing(0)
+    st = F.FPSettings(0, 0, 1, 0, 0, 0.0)
+    assert op.init(F.fp_desc((16, 16), (14, 16), 3), st)
+    inp = torch.zeros((3, 16, 9), dtype=torch.complex64, device="cuda")
+    out = torch.zeros((3, 16, 8), dtype=torch.complex64, device="cuda")
+    filt = torch.ones((16, 8), dtype=torch.float32, device="cuda")
+    assert op.execute(inp, 3, out, 3, filt)
+    with pytest.raises(ValueError, match="output"):
+        op.execute(inp, 3, out[:2], 3, filt)
+    with pytest.raises(ValueError, match="filter"):
+        op.execute(inp, 3, out, 3, filt[:8])
+    with pytest.raises(ValueError, match="input"):
+        op.execute(inp[:, :, :4].contiguous(), 3, out, 3, filt)

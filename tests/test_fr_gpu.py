@@ -245,3 +245,34 @@ def test_boundary_errors():
     op.cleanup()
     assert not op.execute(fft, vol, wts, sp, tab)
     assert op.last_status == _lib.UMPA_ERR_NOT_INITIALIZED
+
+
+def test_python_mirror_refuses_short_strided_or_host_payloads():
+    """Tensors shorter than the Init descriptor, strided views and host tensors never reach
+    the C ABI (it sees plain pointers): ValueError, and nothing is written."""
+    import torch
+    s = settings()
+    op = F.FourierReconstruction(0)
+    dev = torch.device("cuda:0")
+    assert op.init(F.fr_desc(17, 32, 4, 33, 4, 10000), s)
+    vol = torch.zeros((33,) * 3, dtype=torch.complex64, device=dev)
+    wts = torch.zeros((33,) * 3, dtype=torch.float32, device=dev)
+    fft = torch.zeros((4, 32, 17), dtype=torch.complex64, device=dev)
+    sp = torch.zeros(112 * 4, dtype=torch.uint8, device=dev)
+    tab = torch.zeros(10000, dtype=torch.float32, device=dev)
+    with pytest.raises(ValueError, match="volume"):
+        op.execute(fft, vol[:32], wts, sp, tab)
+    with pytest.raises(ValueError, match="weights"):
+        op.execute(fft, vol, wts[:, :20], sp, tab)
+    with pytest.raises(ValueError, match="table"):
+        op.execute(fft, vol, wts, sp, tab[:5000])
+    with pytest.raises(ValueError, match="spaces"):
+        op.execute(fft, vol, wts, sp[:200], tab, n_spaces=4)
+    with pytest.raises(ValueError, match="fft"):
+        op.execute(fft[:, :16], vol, wts, sp, tab)
+    with pytest.raises(ValueError, match="contiguous"):
+        op.execute(fft.transpose(1, 2), vol, wts, sp, tab)
+    with pytest.raises(ValueError, match="CUDA"):
+        op.execute(fft.cpu(), vol, wts, sp, tab)
+    torch.cuda.synchronize()
+    assert float(wts.abs().sum()) == 0.0 and float(vol.abs().sum()) == 0.0

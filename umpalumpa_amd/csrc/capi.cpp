@@ -309,9 +309,10 @@ int umpa_fr_execute_f16(umpa_fr_op *op, const void *fft, const float *scales,
                          table);
 }
 
-int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint64_t fftCount,
-                     const umpa_traverse_space *spaces, uint64_t nSpaces, const float *table,
-                     int useCache, umpa_fr_sched_params *chosen)
+static int fr_autotune_impl(umpa_fr_op *op, const void *fft, const float *scales,
+                            uint64_t fftCount, const umpa_traverse_space *spaces,
+                            uint64_t nSpaces, const float *table, int useCache,
+                            umpa_fr_sched_params *chosen)
 {
   const bool half = scales != nullptr;
   // the checks of execute, with a placeholder for the output (scratch volumes are used)
@@ -415,6 +416,21 @@ int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint6
   op->params[slot] = best;
   if (chosen) *chosen = { best.depthScale, best.priority ? 1 : 0, best.maxParts, best.splitAt };
   return UMPA_OK;
+}
+
+int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint64_t fftCount,
+                     const umpa_traverse_space *spaces, uint64_t nSpaces, const float *table,
+                     int useCache, umpa_fr_sched_params *chosen)
+{
+  // the cache key and the tuning file are handled with std::string and streams: nothing they
+  // throw crosses the C ABI (device buffers are released inside before any of them runs)
+  try {
+    return fr_autotune_impl(op, fft, scales, fftCount, spaces, nSpaces, table, useCache, chosen);
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    return UMPA_ERR_DEVICE;
+  }
 }
 
 int umpa_fr_get_sched_params(umpa_fr_op *op, umpa_fr_sched_params *out)

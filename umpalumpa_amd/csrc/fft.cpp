@@ -2,8 +2,8 @@
 // out-of-place real-to-complex (forward) and complex-to-real (inverse, unnormalised) transforms
 // of 1-, 2- or 3-D images, the upstream step of the FR harness (fr.cpp:134-164).  The plan
 // mirrors FFTCUDA's (fft_cuda.cpp:16-47): contiguous images, rank from the image shape,
-// distances of one padded spatial / frequency image.  Plans are kept per batch count, so an
-// Execute over fewer images than at Init runs a plan of that many.
+// distances of one padded spatial / frequency image.  Plans are kept per batch count (at most
+// eight), so an Execute over fewer images than at Init runs a plan of that many.
 #include <hip/hip_runtime_api.h>
 #include <hipfft/hipfft.h>
 
@@ -35,6 +35,52 @@ int from_fft(hipfftResult r)
 // Size::IsValid (data/size.hpp:43-57) for the 2-D images of the fused operation
 bool valid_2d(const umpa_size &s) { return s.n >= 1 && s.z == 1 && s.y >= 2 && s.x >= 2; }
 
+// Plans by image count.  A count not seen before gets a plan of its own (`make` creates the
+// handle, plans it and sets the stream); past kMaxPlans the stream drains and every plan but
+// the one for the Init count is destroyed, so a caller cycling through counts holds a bounded
+// number of plans and their work areas.
+constexpr size_t kMaxPlans = 8;
+
+template <typename Make>
+int cached_plan(std::map<uint64_t, hipfftHandle> &plans, uint64_t count, uint64_t initCount,
+                hipStream_t stream, Make make, hipfftHandle &out)
+{
+  auto it = plans.find(count);
+  if (it != plans.end()) {
+    out = it->second;
+    return UMPA_OK;
+  }
+  if (plans.size() >= kMaxPlans) {
+    // enqueued transforms may still use the plans' work areas
+    const hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return from_hip(e);
+    for (auto p = plans.begin(); p != plans.end();) {
+      if (p->first == initCount) {
+        ++p;
+        continue;
+      }
+      (void)hipfftDestroy(p->second);
+      p = plans.erase(p);
+    }
+  }
+  hipfftHandle h = nullptr;
+  hipfftResult r = hipfftCreate(&h);
+  if (r == HIPFFT_SUCCESS) r = make(h);
+  if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, stream);
+  if (r != HIPFFT_SUCCESS) {
+    if (h) (void)hipfftDestroy(h);
+    return from_fft(r);
+  }
+  try {
+    plans.emplace(count, h);
+  } catch (const std::bad_alloc &) {
+    (void)hipfftDestroy(h);
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+  out = h;
+  return UMPA_OK;
+}
+
 }  // namespace
 
 struct umpa_fft_op {
@@ -53,11 +99,6 @@ struct umpa_fft_op {
 
   int plan(uint64_t count, hipfftHandle &out)
   {
-    auto it = plans.find(count);
-    if (it != plans.end()) {
-      out = it->second;
-      return UMPA_OK;
-    }
     const umpa_size &s = desc.spatial;
     int n[3];
     int rank;
@@ -79,21 +120,12 @@ struct umpa_fft_op {
     const bool fwd = desc.direction == UMPA_FFT_FORWARD;
     const bool dbl = desc.precision == UMPA_FFT_F64;
     const hipfftType type = fwd ? (dbl ? HIPFFT_D2Z : HIPFFT_R2C) : (dbl ? HIPFFT_Z2D : HIPFFT_C2R);
-    hipfftHandle h = nullptr;
-    hipfftResult r = hipfftCreate(&h);
-    if (r == HIPFFT_SUCCESS) {
+    auto make = [&](hipfftHandle h) {
       size_t work = 0;
-      r = hipfftMakePlanMany(h, rank, n, nullptr, 1, fwd ? spatial : freq, nullptr, 1,
-                             fwd ? freq : spatial, type, (int)count, &work);
-    }
-    if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, stream);
-    if (r != HIPFFT_SUCCESS) {
-      if (h) (void)hipfftDestroy(h);
-      return from_fft(r);
-    }
-    plans[count] = h;
-    out = h;
-    return UMPA_OK;
+      return hipfftMakePlanMany(h, rank, n, nullptr, 1, fwd ? spatial : freq, nullptr, 1,
+                                fwd ? freq : spatial, type, (int)count, &work);
+    };
+    return cached_plan(plans, count, s.n, stream, make, out);
   }
 };
 
@@ -121,28 +153,14 @@ struct umpa_fft_fp_op {
   // batched 1-D R2C over every image row: count * y rows of x reals
   int plan(uint64_t count, hipfftHandle &out)
   {
-    auto it = plans.find(count);
-    if (it != plans.end()) {
-      out = it->second;
-      return UMPA_OK;
-    }
     const umpa_size &s = desc.inSpatial;
     int n[1] = { (int)s.x };
-    hipfftHandle h = nullptr;
-    hipfftResult r = hipfftCreate(&h);
-    if (r == HIPFFT_SUCCESS) {
+    auto make = [&](hipfftHandle h) {
       size_t ws = 0;
-      r = hipfftMakePlanMany(h, 1, n, nullptr, 1, (int)s.x, nullptr, 1, (int)(s.x / 2 + 1),
-                             HIPFFT_R2C, (int)(count * s.y), &ws);
-    }
-    if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, stream);
-    if (r != HIPFFT_SUCCESS) {
-      if (h) (void)hipfftDestroy(h);
-      return from_fft(r);
-    }
-    plans[count] = h;
-    out = h;
-    return UMPA_OK;
+      return hipfftMakePlanMany(h, 1, n, nullptr, 1, (int)s.x, nullptr, 1, (int)(s.x / 2 + 1),
+                                HIPFFT_R2C, (int)(count * s.y), &ws);
+    };
+    return cached_plan(plans, count, s.n, stream, make, out);
   }
 };
 

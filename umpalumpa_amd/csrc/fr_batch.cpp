@@ -310,8 +310,8 @@ int umpa_fr_batch_create(umpa_fr_batch **out, int device, void *stream)
   return UMPA_OK;
 }
 
-int umpa_fr_batch_plan(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
-                       uint32_t chunkSpaces, int strategy, int useGraph)
+static int batch_plan_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                           uint32_t chunkSpaces, int strategy, int useGraph)
 {
   if (!b || chunkSpaces == 0) return UMPA_ERR_INVALID_ARG;
   DeviceGuard g(b->device);
@@ -371,6 +371,18 @@ int umpa_fr_batch_plan(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs
   }
 }
 
+int umpa_fr_batch_plan(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                       uint32_t chunkSpaces, int strategy, int useGraph)
+{
+  try {
+    return batch_plan_impl(b, jobs, nJobs, chunkSpaces, strategy, useGraph);
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    return UMPA_ERR_DEVICE;
+  }
+}
+
 int umpa_fr_batch_execute(umpa_fr_batch *b)
 {
   if (!b) return UMPA_ERR_INVALID_ARG;
@@ -397,8 +409,8 @@ int umpa_fr_batch_execute(umpa_fr_batch *b)
   return from_hip(e);
 }
 
-int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t *begin,
-                       const uint64_t *end, uint32_t nJobs, int strategy)
+static int batch_step_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t *begin,
+                           const uint64_t *end, uint32_t nJobs, int strategy)
 {
   if (!b || !begin || !end) return UMPA_ERR_INVALID_ARG;
   DeviceGuard g(b->device);
@@ -454,9 +466,21 @@ int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t
   }
 }
 
-int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
-                           uint32_t chunkSpaces, int strategy, int useCache,
-                           umpa_fr_sched_params *chosen)
+int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t *begin,
+                       const uint64_t *end, uint32_t nJobs, int strategy)
+{
+  try {
+    return batch_step_impl(b, jobs, begin, end, nJobs, strategy);
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    return UMPA_ERR_DEVICE;
+  }
+}
+
+static int batch_autotune_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                               uint32_t chunkSpaces, int strategy, int useCache,
+                               umpa_fr_sched_params *chosen)
 {
   if (!b || chunkSpaces == 0) return UMPA_ERR_INVALID_ARG;
   DeviceGuard g(b->device);
@@ -493,6 +517,7 @@ int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t n
   hipError_t e = hipStreamSynchronize(b->stream);
   // scratch volumes: the candidates accumulate into these, never into the caller's
   std::vector<void *> scratch;
+  scratch.reserve(2 * infos.size());  // no reallocation between a hipMalloc and its record
   for (auto &j : infos) {
     const size_t n = (size_t)j.rec.vx * j.rec.vy * j.rec.vz;
     void *v = nullptr, *w = nullptr;
@@ -552,6 +577,19 @@ int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t n
   if (e != hipSuccess) return from_hip(e);
   umpa::host::tuning_cache_store(key, best, bestMs);
   return report(best);
+}
+
+int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
+                           uint32_t chunkSpaces, int strategy, int useCache,
+                           umpa_fr_sched_params *chosen)
+{
+  try {
+    return batch_autotune_impl(b, jobs, nJobs, chunkSpaces, strategy, useCache, chosen);
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    return UMPA_ERR_DEVICE;
+  }
 }
 
 int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy, const umpa_fr_sched_params *p)

@@ -43,8 +43,25 @@ def _own(obj, destroy: str) -> None:
     obj._finalizer = weakref.finalize(obj, getattr(_lib.load(), destroy), obj._h)
 
 
-def _ptr(t):
-    return None if t is None else C.c_void_p(t.data_ptr())
+def _ptr(t, what: str = "tensor", nbytes: int = 0):
+    """Device pointer of a contiguous CUDA tensor holding at least `nbytes` bytes (None for
+    None, which the C entry points turn into their own status).  The C ABI takes plain
+    pointers, so a host tensor, a strided view or a buffer shorter than the operation's
+    descriptor would reach the kernels as something else: refused here with ValueError."""
+    if t is None:
+        return None
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{what}: a CUDA tensor is required")
+    if not t.is_contiguous():
+        raise ValueError(f"{what}: the tensor must be contiguous")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise ValueError(f"{what}: {have} bytes, at least {nbytes} needed")
+    return C.c_void_p(t.data_ptr())
+
+
+def _elems(s) -> int:
+    return int(s.x) * int(s.y) * int(s.z)
 
 
 def fr_desc(slice_x: int, slice_y: int, n_slices: int, volume: int, n_spaces: int = 0,
@@ -136,8 +153,11 @@ def finalize(volume, weights, out=None, hermitian: bool = True, stream=None):
     side = volume.shape[-1]
     if out is None:
         out = torch.empty_like(volume)
-    check(_lib.load().umpa_fr_finalize(_ptr(volume), _ptr(weights), side, _ptr(out),
-                                       int(hermitian), _stream_handle(stream)),
+    n = side ** 3
+    check(_lib.load().umpa_fr_finalize(_ptr(volume, "volume", 8 * n),
+                                       _ptr(weights, "weights", 4 * n), side,
+                                       _ptr(out, "out", 8 * n), int(hermitian),
+                                       _stream_handle(stream)),
           "umpa_fr_finalize")
     return out
 
@@ -181,16 +201,30 @@ class FourierReconstruction:
         complex64 [Z, Y, X], weights: float32 [Z, Y, X], spaces: uint8 device tensor of
         112-byte records, table: float32 device tensor."""
         n_sp = spaces.numel() // 112 if n_spaces is None else n_spaces
+        p = self._pointers(fft, scales, spaces, n_sp, table, volume, weights)
         lib = _lib.load()
         if scales is None:
             self.last_status = lib.umpa_fr_execute(
-                self._h, _ptr(fft), fft.shape[0], _ptr(volume), _ptr(weights), _ptr(spaces),
-                n_sp, _ptr(table))
+                self._h, p["fft"], fft.shape[0], p["volume"], p["weights"], p["spaces"], n_sp,
+                p["table"])
         else:
             self.last_status = lib.umpa_fr_execute_f16(
-                self._h, _ptr(fft), _ptr(scales), fft.shape[0], _ptr(volume), _ptr(weights),
-                _ptr(spaces), n_sp, _ptr(table))
+                self._h, p["fft"], p["scales"], fft.shape[0], p["volume"], p["weights"],
+                p["spaces"], n_sp, p["table"])
         return self.last_status == 0
+
+    def _pointers(self, fft, scales, spaces, n_sp, table, volume=None, weights=None) -> dict:
+        """Checked device pointers, each tensor at least as large as the Init descriptor says
+        (a count above Init's is left to the C entry point, which refuses it)."""
+        d = self.desc
+        n = 0 if fft is None else min(fft.shape[0], d.fft.n if d is not None else 0)
+        px = 0 if d is None else int(d.fft.x) * int(d.fft.y) * (4 if scales is not None else 8)
+        vox = 0 if d is None else _elems(d.volume)
+        return {"fft": _ptr(fft, "fft", n * px), "scales": _ptr(scales, "scales", 4 * n),
+                "spaces": _ptr(spaces, "spaces", 112 * n_sp),
+                "table": _ptr(table, "table", 0 if d is None else 4 * int(d.blobTable)),
+                "volume": _ptr(volume, "volume", 8 * vox),
+                "weights": _ptr(weights, "weights", 4 * vox)}
 
     def autotune(self, fft, spaces, table=None, n_spaces=None, scales=None,
                  use_cache: bool = True) -> SchedParams:
@@ -198,8 +232,9 @@ class FourierReconstruction:
         the fastest for the current strategy (umpa_fr_autotune)."""
         n_sp = spaces.numel() // 112 if n_spaces is None else n_spaces
         out = SchedParams()
-        check(_lib.load().umpa_fr_autotune(self._h, _ptr(fft), _ptr(scales), fft.shape[0],
-                                           _ptr(spaces), n_sp, _ptr(table), int(use_cache),
+        p = self._pointers(fft, scales, spaces, n_sp, table)
+        check(_lib.load().umpa_fr_autotune(self._h, p["fft"], p["scales"], fft.shape[0],
+                                           p["spaces"], n_sp, p["table"], int(use_cache),
                                            C.byref(out)), "umpa_fr_autotune")
         return out
 
@@ -234,10 +269,17 @@ class BatchJob:
         n, hy, hx = self.fft.shape[0], self.fft.shape[1], self.fft.shape[2]
         v = self.volume.shape[-1]
         desc = fr_desc(hx, hy, n, v, self.n_spaces, self.n_table)
-        return FRJob(desc, self.settings, self.fft.data_ptr(),
-                     None if self.scales is None else self.scales.data_ptr(),
-                     self.volume.data_ptr(), self.weights.data_ptr(), self.spaces.data_ptr(),
-                     None if self.table is None else self.table.data_ptr())
+        px = 4 if self.scales is not None else 8
+
+        def ptr(t, what, nbytes):
+            p = _ptr(t, what, nbytes)
+            return None if p is None else p.value
+
+        return FRJob(desc, self.settings, ptr(self.fft, "fft", n * hy * hx * px),
+                     ptr(self.scales, "scales", 4 * n), ptr(self.volume, "volume", 8 * v ** 3),
+                     ptr(self.weights, "weights", 4 * v ** 3),
+                     ptr(self.spaces, "spaces", 112 * self.n_spaces),
+                     ptr(self.table, "table", 4 * self.n_table))
 
 
 class FRBatch:
@@ -341,6 +383,7 @@ class FourierTransform:
                                           _stream_handle(self._stream)), "umpa_fft_create")
         _own(self, "umpa_fft_destroy")
         self.last_status = 0
+        self.desc = None
 
     def init(self, spatial, n: int, direction: int = FFT_FORWARD,
              precision: int = FFT_F32) -> bool:
@@ -348,10 +391,23 @@ class FourierTransform:
         x, y, z = (tuple(spatial) + (1, 1))[:3]
         d = FFTDesc(Size(x, y, z, n), direction, precision)
         self.last_status = _lib.load().umpa_fft_init(self._h, C.byref(d))
+        self.desc = d if self.last_status == 0 else None
         return self.last_status == 0
 
     def execute(self, inp, out, count: int) -> bool:
-        self.last_status = _lib.load().umpa_fft_execute(self._h, _ptr(inp), _ptr(out), count)
+        d = self.desc
+        real = freq = 0
+        if d is not None:
+            m = min(count, int(d.spatial.n))
+            s = d.spatial
+            w = 8 if d.precision == FFT_F64 else 4
+            real = m * _elems(s) * w
+            freq = m * (int(s.x) // 2 + 1) * int(s.y) * int(s.z) * 2 * w
+        src, dst = (real, freq) if d is None or d.direction == FFT_FORWARD else (freq, real)
+        if inp is not None and out is not None and inp.data_ptr() == out.data_ptr():
+            dst = 0  # in place: the C entry point refuses it (UMPA_ERR_UNSUPPORTED)
+        self.last_status = _lib.load().umpa_fft_execute(self._h, _ptr(inp, "input", src),
+                                                        _ptr(out, "output", dst), count)
         return self.last_status == 0
 
     def synchronize(self) -> None:
@@ -375,16 +431,24 @@ class FourierTransformProcessing:
                                              _stream_handle(self._stream)), "umpa_fft_fp_create")
         _own(self, "umpa_fft_fp_destroy")
         self.last_status = 0
+        self.desc = None
 
     def init(self, desc: FPDesc, settings: FPSettings) -> bool:
         self.last_status = _lib.load().umpa_fft_fp_init(self._h, C.byref(desc),
                                                         C.byref(settings))
+        self.desc = desc if self.last_status == 0 else None
         return self.last_status == 0
 
     def execute(self, images, count: int, out, filt=None) -> bool:
         """images: float32 [count, y, x]; out: complex64 [count, out.y, out.x]."""
-        self.last_status = _lib.load().umpa_fft_fp_execute(self._h, _ptr(images), count,
-                                                           _ptr(out), _ptr(filt))
+        d = self.desc
+        m = 0 if d is None else min(count, int(d.inSpatial.n))
+        img = 0 if d is None else 4 * m * _elems(d.inSpatial)
+        res = 0 if d is None else 8 * m * _elems(d.out)
+        flt = 0 if d is None else 4 * int(d.out.x) * int(d.out.y)
+        self.last_status = _lib.load().umpa_fft_fp_execute(
+            self._h, _ptr(images, "images", img), count, _ptr(out, "out", res),
+            _ptr(filt, "filter", flt))
         return self.last_status == 0
 
     def synchronize(self) -> None:
@@ -404,14 +468,22 @@ class FourierProcessing:
               "umpa_fp_create")
         _own(self, "umpa_fp_destroy")
         self.last_status = 0
+        self.desc = None
 
     def init(self, desc: FPDesc, settings: FPSettings) -> bool:
         self.last_status = _lib.load().umpa_fp_init(self._h, C.byref(desc), C.byref(settings))
+        self.desc = desc if self.last_status == 0 else None
         return self.last_status == 0
 
     def execute(self, inp, in_count, out, out_count, filt=None) -> bool:
-        self.last_status = _lib.load().umpa_fp_execute(self._h, _ptr(inp), in_count, _ptr(out),
-                                                       out_count, _ptr(filt))
+        d = self.desc
+        src = 0 if d is None else 8 * min(in_count, int(d.in_.n)) * _elems(d.in_)
+        # the kernel writes the first in_count output images
+        dst = 0 if d is None else 8 * min(in_count, out_count, int(d.out.n)) * _elems(d.out)
+        flt = 0 if d is None else 4 * int(d.out.x) * int(d.out.y)
+        self.last_status = _lib.load().umpa_fp_execute(
+            self._h, _ptr(inp, "input", src), in_count, _ptr(out, "output", dst), out_count,
+            _ptr(filt, "filter", flt))
         return self.last_status == 0
 
     def synchronize(self) -> None:
