@@ -174,7 +174,8 @@ struct Contribution {
 // unrolled NB x NB box, which only rounding at the blob edge can produce.
 template <int kMode, int kOrder, bool kHalf, bool kFastMath>
 __device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buffer_rsrc_t rsrc,
-                                       const float *tbl, float qx, float qy, float zSqr, float dw)
+                                       const float *tbl, float qx, float qy, float zSqr, float dw,
+                                       uint32_t sliceOff = 0)
 {
   const umpa_fr_constants &c = a.c;
   const float r = c.cBlobRadius, r2 = r * r;
@@ -203,7 +204,7 @@ __device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buff
         wb = blob_weight<kMode, kOrder>(d2, c);
       }
       float weight = wb * dw;
-      float2 f = load_pixel<kHalf>(rsrc, (uint32_t)((i * a.xSize + j) * (kHalf ? 4 : 8)));
+      float2 f = load_pixel<kHalf>(rsrc, sliceOff + (uint32_t)((i * a.xSize + j) * (kHalf ? 4 : 8)));
       o.w += weight;
       if constexpr (kFastMath) {
         o.re = __builtin_fmaf(f.x, weight, o.re);
@@ -218,9 +219,9 @@ __device__ Contribution gather_generic(const FrJob &a, int nTable, __amdgpu_buff
 }
 
 // load_row for the exact FP32 Kaiser-mode kernels: a row whose 16-byte loads straddle the slice
-// end is read pixel by pixel.  This compiler merges those pixel loads back into the same
-// 16-byte loads, so both sides load alike; the branch only splits the rows into scheduling
-// regions, the form these kernels were measured and kept in.
+// end (byte `sliceBytes` of the resource) is read pixel by pixel.  This compiler merges those
+// pixel loads back into the same 16-byte loads, so both sides load alike; the branch only
+// splits the rows into scheduling regions, the form these kernels were measured and kept in.
 template <bool kHalf, int kNB>
 __device__ __forceinline__ void load_row_regions(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff,
                                                  uint32_t sliceBytes, Pixel<kHalf> (&px)[kNB])
@@ -286,7 +287,7 @@ template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnit
 __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
                                            __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
                                            float qx, float qy, float zSqr, float dw,
-                                           Contribution &o)
+                                           Contribution &o, uint32_t sliceOff = 0)
 {
   const umpa_fr_constants &c = a.c;
   const float r = c.cBlobRadius, r2 = r * r;
@@ -315,8 +316,10 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
     yz[ii] = (ii <= ny) ? dy * dy + zSqr : inf;
   }
   const uint32_t rowStride = (uint32_t)(a.xSize * kPix);
-  const uint32_t base = (uint32_t)(minY * a.xSize + minX) * kPix;
-  const uint32_t sliceBytes = (uint32_t)(a.sliceElems * kPix);
+  // sliceOff: the slice's byte offset within the buffer resource (0 when the resource covers
+  // the slice alone)
+  const uint32_t base = sliceOff + (uint32_t)(minY * a.xSize + minX) * kPix;
+  const uint32_t sliceEnd = sliceOff + (uint32_t)(a.sliceElems * kPix);
   float w = 0.f, vr = 0.f, vi = 0.f;
   if constexpr (kNB <= 4) {
     // Whole candidate rows, summed in the reference's order (rows, then columns).  Candidates
@@ -353,7 +356,7 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
       if constexpr (kRegions) {
         load_row<kHalf, kNB>(rsrc, rowOff, px);
       } else if constexpr (!kHalf && !kFastMath) {
-        load_row_regions<kHalf, kNB>(rsrc, rowOff, sliceBytes, px);
+        load_row_regions<kHalf, kNB>(rsrc, rowOff, sliceEnd, px);
       } else {
         load_row<kHalf, kNB>(rsrc, rowOff, px);
       }
@@ -694,6 +697,229 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
   }
 }
 
+// ---- exact FP32 lookup insertion with cross-space gather batches -----------------------------
+//
+// In fr_insert_kernel a space's queue of slab voxels is gathered 64 at a time, so the last,
+// partly filled batch of every (brick, space) pair leaves lanes idle: two thirds of the gather
+// lanes do work at box 256.  This variant, for the exact strategy with the table in LDS (the
+// one with LDS to spare at 12 wavefronts per CU), evaluates the predicate per space as before
+// but appends the hits (voxel, space ordinal, q, slice offset, weight) to a per-wave ring in
+// LDS and gathers whenever 64 are waiting, across spaces.  All slices are read through one
+// buffer resource over the chunk.  A voxel still receives its contributions in space order:
+// entries are appended in space order, and a batch adds its contributions space by space.
+constexpr int kStagedWaves = 12;
+constexpr int kStage = 128;  // ring entries per wave: a full batch plus one space's hits
+// largest table the staged kernel's LDS holds beside its per-wave state (160 KiB in all)
+constexpr int kStagedMaxTable =
+  (160 * 1024 - kStagedWaves * (int)(sizeof(WaveScratch) + 6 * 4 * kStage)) / 4 - 1;
+
+struct WaveStage {
+  float qx[kStage], qy[kStage], zs[kStage], dw[kStage];
+  uint32_t off[kStage];  // byte offset of the entry's slice in the chunk
+  uint32_t key[kStage];  // voxel index in the brick (bits 0-15), space ordinal (bits 16-31)
+};
+
+template <int kNB>
+__device__ __forceinline__ void staged_gather(const FrJob &a, int nTable,
+                                              __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
+                                              WaveScratch &ws, const WaveStage &st, int lane,
+                                              int head, int n)
+{
+  const int e = (head + lane) & (kStage - 1);
+  const bool act = lane < n;
+  const float qx = act ? st.qx[e] : 0.f, qy = act ? st.qy[e] : 0.f, zs = act ? st.zs[e] : 0.f;
+  const float dw = act ? st.dw[e] : 1.f;
+  const uint32_t off = act ? st.off[e] : 0u, key = act ? st.key[e] : 0u;
+  Contribution o{ 0.f, 0.f, 0.f };
+  if (act) {
+    // a weight of 1 multiplies nothing; with another weight the product wb * dw is the
+    // reference's, and wb * 1 = wb, so the choice per batch changes no bit
+    bool fits;
+    if (__all(dw == 1.f)) {
+      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true>(a, nTable, rsrc, tbl, qx,
+                                                                       qy, zs, dw, o, off);
+    } else {
+      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true>(a, nTable, rsrc, tbl, qx,
+                                                                        qy, zs, dw, o, off);
+    }
+    if (!fits)
+      o = gather_generic<kModeLookup, 0, false, false>(a, nTable, rsrc, tbl, qx, qy, zs, dw, off);
+  }
+  // add space by space (the lowest remaining lane holds the earliest space): within a space
+  // every voxel appears once
+  unsigned long long rem = __ballot(act);
+  while (rem) {
+    const int first = __builtin_ctzll(rem);
+    const uint32_t ord = (uint32_t)__builtin_amdgcn_readlane((int)(key >> 16), first);
+    const unsigned long long m = __ballot(act && (key >> 16) == ord) & rem;
+    if ((m >> lane) & 1ull) {
+      const int v = (int)(key & 0xffffu);
+      ws.accR[v] = ws.accR[v] + o.re;
+      ws.accI[v] = ws.accI[v] + o.im;
+      ws.accW[v] = ws.accW[v] + o.w;
+    }
+    rem &= ~m;
+  }
+}
+
+template <int kNB>
+__global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJob a, FrLaunch l)
+{
+  constexpr int kWaves = kStagedWaves;
+  constexpr int kThreads = 64 * kWaves;
+  constexpr int kPix = 8;
+  extern __shared__ float s_table[];
+  __shared__ WaveScratch s_wave[kWaves];
+  __shared__ WaveStage s_stage[kWaves];
+  const int nTable = l.nTable;
+  for (int i = threadIdx.x; i < nTable; i += kThreads) s_table[i] = l.table[i];
+  if (threadIdx.x == 0) s_table[nTable] = 0.f;
+  __syncthreads();
+  const float *tbl = s_table;
+  const int lane = threadIdx.x & 63;
+  WaveScratch &ws = s_wave[threadIdx.x >> 6];
+  WaveStage &st = s_stage[threadIdx.x >> 6];
+  const int lx = lane & (kEdge - 1), ly = lane / kEdge;
+  constexpr float kHalfEdge = 0.5f * (kEdge - 1);
+  // one resource over the chunk's slices (the host keeps it under 2 GiB); a candidate row
+  // read past its slice's end reads the next slice or zeros, always at weight 0
+  const uint32_t sliceBytes = (uint32_t)(a.sliceElems * kPix);
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+    (void *)a.fft, 0, (int)((uint32_t)a.nProj * sliceBytes), kBufferConfig);
+  const umpa_fr_constants &c = a.c;
+  const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
+  const float r2 = c.cBlobRadius * c.cBlobRadius;
+
+  for (;;) {
+    int b = 0;
+    if (lane == 0) b = atomicAdd(l.counter, 1);
+    b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
+    if (b >= l.nItems) break;
+    const int4 bo = make_uniform(reinterpret_cast<const int4 *>(l.items)[b]);
+    // work item: brick 8 x 8 x depth at (bo.x, bo.y, bo.z); the exact schedules never split
+    const int bz = bo.z & 0xffff;
+    const int sBegin = a.spaceBegin, sEnd = a.spaceEnd;
+    if (sBegin >= sEnd) continue;
+    const int depth = bo.w & 15;
+    const bool heavy = (bo.w >> 4) & 1;
+    if (heavy) __builtin_amdgcn_s_setprio(2);
+    const int x = bo.x + lx, y = bo.y + ly;
+    const bool colIn = x < a.vx && y < a.vy;
+    const int nz = min(depth, a.vz - bz);
+    const float px = (float)(x - cxi), py = (float)(y - cyi);
+    for (int k = 0; k < nz; ++k) {
+      const int li = k * 64 + lane;
+      float vr = 0.f, vi = 0.f, vw = 0.f;
+      if (colIn) {
+        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
+        const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
+        vr = t.x;
+        vi = t.y;
+        vw = a.weights[gi];
+      }
+      ws.accR[li] = vr;
+      ws.accI[li] = vi;
+      ws.accW[li] = vw;
+    }
+    wave_sync();
+    const float hz = 0.5f * (float)(depth - 1);
+    const float bcx = (float)(bo.x - cxi) + kHalfEdge;
+    const float bcy = (float)(bo.y - cyi) + kHalfEdge;
+    const float bcz = (float)(bz - cyi) + hz;
+    const float bnorm = sqrtf(bcx * bcx + bcy * bcy + bcz * bcz);
+    const float dmin = fmaxf(bnorm - sqrtf(2.f * kHalfEdge * kHalfEdge + hz * hz), 0.f);
+    const float eps = 0.02f + 1e-5f * bnorm;
+    int head = 0, waiting = 0;  // the ring's oldest entry and its number of entries
+    uint32_t ordinal = 0;       // spaces taken so far by this work item (mod 2^16)
+    for (int base = sBegin; base < sEnd; base += 64) {
+      const int si = base + lane;
+      bool pass = false;
+      if (si < sEnd) {
+        const umpa_traverse_space &sp = a.spaces[si];
+        const float(*t)[3] = sp.transformInv;
+        const float qz = t[2][0] * bcx + t[2][1] * bcy + t[2][2] * bcz;
+        const float ez = kHalfEdge * (fabsf(t[2][0]) + fabsf(t[2][1])) + hz * fabsf(t[2][2]);
+        const float qx = t[0][0] * bcx + t[0][1] * bcy + t[0][2] * bcz;
+        const float ex = kHalfEdge * (fabsf(t[0][0]) + fabsf(t[0][1])) + hz * fabsf(t[0][2]);
+        pass = fabsf(qz) <= a.cullHalfWidth + ez + eps && qx >= -a.cullHalfSpace - ex - eps &&
+               dmin * dmin <= sp.maxDistanceSqr * 1.0001f + eps && sp.projectionIndex >= 0 &&
+               sp.projectionIndex < a.nProj && sp.dir >= 0 && sp.dir <= 2;
+      }
+      unsigned long long mask = __ballot(pass);
+      while (mask) {
+        const int k0 = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const umpa_traverse_space sp = a.spaces[__builtin_amdgcn_readfirstlane(base + k0)];
+        const uint32_t sliceOff =
+          (uint32_t)__builtin_amdgcn_readfirstlane(sp.projectionIndex) * sliceBytes;
+        const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
+                    t22 = sp.transformInv[2][2];
+        // phase 1: the slab test, as in fr_insert_kernel
+        const float cz = t20 * px + t21 * py;
+        int cnt = 0;
+        for (int k = 0; k < nz; ++k) {
+          const float pz = (float)(bz + k - cyi);
+          const float qz = cz + t22 * pz;
+          const bool ok = colIn && !(qz * qz > r2);
+          const unsigned long long m = __ballot(ok);
+          if (ok) ws.queue[cnt + (int)lanes_below(m)] = (uint16_t)(k * 64 + lane);
+          cnt += __popcll(m);
+        }
+        if (cnt == 0) continue;
+        wave_sync();
+        // phase 2: the rest of the predicate; hits go to the ring, gathered 64 at a time
+        for (int t0 = 0; t0 < cnt; t0 += 64) {
+          const int item = t0 + lane;
+          const bool act = item < cnt;
+          const int v = act ? (int)ws.queue[item] : 0;
+          const int vl = v & 63, vk = v >> 6;
+          const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bz + vk;
+          const float pxv = (float)(xv - cxi), pyv = (float)(yv - cyi), pzv = (float)(zv - cyi);
+          const float dsq = pxv * pxv + pyv * pyv + pzv * pzv;
+          float qx, qy, zs;
+          const bool hit =
+            act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, dsq, qx, qy, zs);
+          const unsigned long long hm = __ballot(hit);
+          if (hit) {
+            const int e = (head + waiting + (int)lanes_below(hm)) & (kStage - 1);
+            st.qx[e] = qx;
+            st.qy[e] = qy;
+            st.zs[e] = zs;
+            st.dw[e] = sp.weight;
+            st.off[e] = sliceOff;
+            st.key[e] = (uint32_t)v | (ordinal << 16);
+          }
+          waiting += __popcll(hm);
+          if (waiting >= 64) {
+            wave_sync();
+            staged_gather<kNB>(a, nTable, rsrc, tbl, ws, st, lane, head, 64);
+            wave_sync();
+            head = (head + 64) & (kStage - 1);
+            waiting -= 64;
+          }
+        }
+        ordinal = (ordinal + 1) & 0xffffu;
+        wave_sync();
+      }
+    }
+    if (waiting > 0) {
+      wave_sync();
+      staged_gather<kNB>(a, nTable, rsrc, tbl, ws, st, lane, head, waiting);
+      wave_sync();
+    }
+    for (int k = 0; k < nz; ++k) {
+      if (colIn) {
+        const int li = k * 64 + lane;
+        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
+        reinterpret_cast<float2 *>(a.volume)[gi] = make_float2(ws.accR[li], ws.accI[li]);
+        a.weights[gi] = ws.accW[li];
+      }
+    }
+    wave_sync();
+    if (heavy) __builtin_amdgcn_s_setprio(0);
+  }
+}
+
 __global__ void fr_reset_counter(int *counter) { *counter = 0; }
 
 // ---- host-side dispatch ------------------------------------------------------------------
@@ -761,13 +987,33 @@ static const void *pick_kernel(const FrVariant &v, int nTable)
   }
 }
 
+// The exact FP32 lookup kernel with cross-space gather batches, for the variants it covers
+// (the launch also needs the chunk's slices under 2 GiB, one buffer resource).
+static const void *pick_staged(const FrVariant &v, int nTable)
+{
+  if (v.mode != kModeLookup || v.half || v.fastMath || v.count || v.batched) return nullptr;
+  if (!table_in_lds(v, nTable) || nTable > kStagedMaxTable) return nullptr;
+  if (v.nb == 3) return (const void *)fr_insert_staged_kernel<3>;
+  if (v.nb == 4) return (const void *)fr_insert_staged_kernel<4>;
+  return nullptr;
+}
+
+static bool staged_fits(const FrJob &job)
+{
+  return (uint64_t)job.nProj * (uint64_t)job.sliceElems * 8u <= 0x7fff0000ull;
+}
+
 hipError_t prepare_fr_insert(const FrVariant &v, int nTable)
 {
   const void *k = pick_kernel(v, nTable);
   if (!k) return hipErrorInvalidValue;
   if (!table_in_lds(v, nTable)) return hipSuccess;
-  return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)((nTable + 1) * sizeof(float)));
+  const int bytes = (int)((nTable + 1) * sizeof(float));
+  hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  const void *staged = pick_staged(v, nTable);
+  if (e == hipSuccess && staged)
+    e = hipFuncSetAttribute(staged, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  return e;
 }
 
 hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v, int numCUs,
@@ -777,7 +1023,12 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
   if (!k) return hipErrorInvalidValue;
   const size_t shmem = table_in_lds(v, l.nTable) ? (size_t)(l.nTable + 1) * sizeof(float) : 0;
   // persistent workgroups: the per-wave LDS state allows one per CU
-  const int waves = fr_waves(v.mode, v.half, v.fastMath);
+  int waves = fr_waves(v.mode, v.half, v.fastMath);
+  const void *staged = pick_staged(v, l.nTable);
+  if (staged && staged_fits(job)) {
+    k = staged;
+    waves = kStagedWaves;
+  }
   int blocks = (l.nItems + waves - 1) / waves;
   if (blocks > numCUs) blocks = numCUs;
   if (blocks < 1) blocks = 1;
