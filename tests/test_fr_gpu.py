@@ -154,6 +154,33 @@ def test_box1024_bit_exact_few_slices():
     assert w.max() > 0
 
 
+def test_exact_chunks_over_2gib_match_smaller_chunks():
+    """The exact kernel with cross-space gather batches reads a chunk's slices through one
+    buffer resource, so chunks of more than 2 GiB of slices run the per-slice-resource kernel.
+    Two box-1024 slices inserted from a 2-slice buffer and from a 520-slice buffer (4.2 MB
+    slices, 2.2 GB in all) give the same bits."""
+    import torch
+    s = settings()
+    fft, spaces, _, table = problem(1024, 2, s, seed=17)
+    dev = torch.device("cuda:0")
+    d_tab = torch.from_numpy(table).to(dev)
+    d_sp = torch.from_numpy(np.ascontiguousarray(spaces).view(np.uint8)).to(dev)
+    big = torch.zeros((520,) + fft.shape[1:], dtype=torch.complex64, device=dev)
+    big[:2] = torch.from_numpy(np.ascontiguousarray(fft)).to(dev)
+    out = []
+    for buf in (big[:2], big):
+        desc = F.fr_desc(fft.shape[2], fft.shape[1], buf.shape[0], 1025, len(spaces), table.size)
+        op = F.FourierReconstruction(0)
+        assert op.init(desc, s), op.last_status
+        vol = torch.zeros((1025,) * 3, dtype=torch.complex64, device=dev)
+        wts = torch.zeros((1025,) * 3, dtype=torch.float32, device=dev)
+        assert op.execute(buf, vol, wts, d_sp, d_tab), op.last_status
+        op.synchronize()
+        out.append((torch.view_as_real(vol).view(torch.int32), wts.view(torch.int32)))
+    assert float(out[0][1].view(torch.float32).max()) > 0
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
 def test_box256_weights_do_not_depend_on_values():
     """Weights depend on geometry only; volume is linear in the slice values."""
     s = settings()
