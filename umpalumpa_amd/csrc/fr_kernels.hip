@@ -485,6 +485,128 @@ __device__ __forceinline__ T make_uniform(const T &v)
   return out;
 }
 
+// ---- the brick's steps shared by the insertion kernels ----------------------------------------
+constexpr float kHalfEdge = 0.5f * (kEdge - 1);
+
+// Brick of the work item and the lane's column of it.
+struct BrickCtx {
+  int x0, y0, z0;     // brick origin
+  int nz;             // levels inside the volume (wave-uniform)
+  int x, y;           // the lane's column
+  bool colIn;         // the column lies inside the volume
+  float px, py;       // the column in centred coordinates
+  float bcx, bcy, bcz, hz, dmin, eps;  // brick centre, half depth and the cull's margins
+};
+
+__device__ __forceinline__ BrickCtx make_brick(const FrJob &a, const int4 &bo, int lane)
+{
+  BrickCtx b;
+  const umpa_fr_constants &c = a.c;
+  const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
+  const int depth = bo.w & 15;
+  b.x0 = bo.x;
+  b.y0 = bo.y;
+  b.z0 = bo.z & 0xffff;
+  b.x = bo.x + (lane & (kEdge - 1));
+  b.y = bo.y + lane / kEdge;
+  b.colIn = b.x < a.vx && b.y < a.vy;
+  b.nz = min(depth, a.vz - b.z0);
+  b.px = (float)(b.x - cxi);
+  b.py = (float)(b.y - cyi);
+  // brick centre in centred volume coordinates, half extents (3.5, 3.5, (depth - 1) / 2)
+  b.hz = 0.5f * (float)(depth - 1);
+  b.bcx = (float)(bo.x - cxi) + kHalfEdge;
+  b.bcy = (float)(bo.y - cyi) + kHalfEdge;
+  b.bcz = (float)(b.z0 - cyi) + b.hz;
+  const float bnorm = sqrtf(b.bcx * b.bcx + b.bcy * b.bcy + b.bcz * b.bcz);
+  b.dmin = fmaxf(bnorm - sqrtf(2.f * kHalfEdge * kHalfEdge + b.hz * b.hz), 0.f);
+  b.eps = 0.02f + 1e-5f * bnorm;
+  return b;
+}
+
+// The brick's running sums start from the volume (or from zero for a split part).
+__device__ __forceinline__ void load_brick(WaveScratch &ws, const FrJob &a, const BrickCtx &b,
+                                           int lane, bool fromZero)
+{
+  for (int k = 0; k < b.nz; ++k) {
+    const int li = k * 64 + lane;
+    float vr = 0.f, vi = 0.f, vw = 0.f;
+    if (!fromZero && b.colIn) {
+      const int64_t gi = ((int64_t)(b.z0 + k) * a.vy + b.y) * a.vx + b.x;
+      const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
+      vr = t.x;
+      vi = t.y;
+      vw = a.weights[gi];
+    }
+    ws.accR[li] = vr;
+    ws.accI[li] = vi;
+    ws.accW[li] = vw;
+  }
+  wave_sync();
+}
+
+// The brick's sums back to the volume: stored, or added with atomics for a split part.
+__device__ __forceinline__ void store_brick(const WaveScratch &ws, const FrJob &a,
+                                            const BrickCtx &b, int lane, bool split)
+{
+  for (int k = 0; k < b.nz; ++k) {
+    if (b.colIn) {
+      const int li = k * 64 + lane;
+      const int64_t gi = ((int64_t)(b.z0 + k) * a.vy + b.y) * a.vx + b.x;
+      if (split) {
+        float *vol = reinterpret_cast<float *>(a.volume);
+        if (ws.accW[li] != 0.f || ws.accR[li] != 0.f || ws.accI[li] != 0.f) {
+          atomicAdd(vol + 2 * gi, ws.accR[li]);
+          atomicAdd(vol + 2 * gi + 1, ws.accI[li]);
+          atomicAdd(a.weights + gi, ws.accW[li]);
+        }
+      } else {
+        reinterpret_cast<float2 *>(a.volume)[gi] = make_float2(ws.accR[li], ws.accI[li]);
+        a.weights[gi] = ws.accW[li];
+      }
+    }
+  }
+  wave_sync();
+}
+
+// Cull of space `si` (one per lane) against the brick: slab half-width, half-space and sphere
+// with the brick's extents as margins; spaces naming a slice that does not exist never pass.
+__device__ __forceinline__ bool cull_space(const FrJob &a, const BrickCtx &b, int si, int sEnd)
+{
+  if (si >= sEnd) return false;
+  const umpa_traverse_space &sp = a.spaces[si];
+  const float(*t)[3] = sp.transformInv;
+  const float qz = t[2][0] * b.bcx + t[2][1] * b.bcy + t[2][2] * b.bcz;
+  const float ez = kHalfEdge * (fabsf(t[2][0]) + fabsf(t[2][1])) + b.hz * fabsf(t[2][2]);
+  const float qx = t[0][0] * b.bcx + t[0][1] * b.bcy + t[0][2] * b.bcz;
+  const float ex = kHalfEdge * (fabsf(t[0][0]) + fabsf(t[0][1])) + b.hz * fabsf(t[0][2]);
+  return fabsf(qz) <= a.cullHalfWidth + ez + b.eps && qx >= -a.cullHalfSpace - ex - b.eps &&
+         b.dmin * b.dmin <= sp.maxDistanceSqr * 1.0001f + b.eps && sp.projectionIndex >= 0 &&
+         sp.projectionIndex < a.nProj && sp.dir >= 0 && sp.dir <= 2;
+}
+
+// Phase 1: the slab test q.z^2 <= slab2 of the column's nz voxels; the voxels that pass go to
+// the wave's queue (index k * 64 + lane), their number is returned.  qz = (t20 px + t21 py) +
+// t22 pz is the reference's left-to-right sum, so the shared partial sum changes no bit.
+__device__ __forceinline__ int slab_queue(WaveScratch &ws, const FrJob &a, const BrickCtx &b,
+                                          const umpa_traverse_space &sp, float slab2, int lane)
+{
+  const int cyi = a.c.cMaxVolumeIndexYZ / 2;
+  const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
+              t22 = sp.transformInv[2][2];
+  const float cz = t20 * b.px + t21 * b.py;
+  int cnt = 0;
+  for (int k = 0; k < b.nz; ++k) {
+    const float pz = (float)(b.z0 + k - cyi);
+    const float qz = cz + t22 * pz;
+    const bool ok = b.colIn && !(qz * qz > slab2);
+    const unsigned long long m = __ballot(ok);
+    if (ok) ws.queue[cnt + (int)lanes_below(m)] = (uint16_t)(k * 64 + lane);
+    cnt += __popcll(m);
+  }
+  return cnt;
+}
+
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount, bool kTableLds,
           bool kBatched>
 __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_insert_kernel(FrJob job0, FrLaunch l)
@@ -502,8 +624,6 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
   const float *tbl = (kMode == kModeLookup && kTableLds) ? s_table : l.table;
   const int lane = threadIdx.x & 63;
   WaveScratch &ws = s_wave[threadIdx.x >> 6];
-  const int lx = lane & (kEdge - 1), ly = lane / kEdge;
-  constexpr float kHalfEdge = 0.5f * (kEdge - 1);
   constexpr int kPix = kHalf ? 4 : 8;
 
   for (;;) {
@@ -516,10 +636,8 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
     // bricks run at raised wave priority; a brick split into `parts` slice ranges accumulates
     // its range from zero and merges the partial sums with atomics (fast-math schedules only)
     const FrJob a = kBatched ? make_uniform(l.jobs[bo.z >> 16]) : job0;
-    const int bz = bo.z & 0xffff;
     const int s0 = a.spaceBegin, s1 = a.spaceEnd;
     if (s0 >= s1) continue;
-    const int depth = bo.w & 15;
     const bool heavy = (bo.w >> 4) & 1;
     const int parts = (bo.w >> 8) & 255, part = (bo.w >> 16) & 255;
     const bool split = parts > 1;
@@ -533,50 +651,13 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
     // the nearest-neighbour traversal the visited voxel lies within 1.5 of the plane
     const float slab2 = kMode == kModeNearest ? 2.56f : r2;
     if (heavy) __builtin_amdgcn_s_setprio(2);
-    // the lane owns the z column (x, y, bz .. bz + depth - 1) of the brick
-    const int x = bo.x + lx, y = bo.y + ly;
-    const bool colIn = x < a.vx && y < a.vy;
-    const int nz = min(depth, a.vz - bz);
-    const float px = (float)(x - cxi), py = (float)(y - cyi);
-    for (int k = 0; k < nz; ++k) {
-      const int li = k * 64 + lane;
-      float vr = 0.f, vi = 0.f, vw = 0.f;
-      if (!split && colIn) {
-        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
-        const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
-        vr = t.x;
-        vi = t.y;
-        vw = a.weights[gi];
-      }
-      ws.accR[li] = vr;
-      ws.accI[li] = vi;
-      ws.accW[li] = vw;
-    }
-    wave_sync();
+    // the lane owns the z column (x, y, z0 .. z0 + depth - 1) of the brick
+    const BrickCtx bk = make_brick(a, bo, lane);
+    const int bz = bk.z0;
+    load_brick(ws, a, bk, lane, split);
     unsigned long long nupd = 0;
-    // brick centre in centred volume coordinates, half extents (3.5, 3.5, (depth - 1) / 2)
-    const float hz = 0.5f * (float)(depth - 1);
-    const float bcx = (float)(bo.x - cxi) + kHalfEdge;
-    const float bcy = (float)(bo.y - cyi) + kHalfEdge;
-    const float bcz = (float)(bz - cyi) + hz;
-    const float bnorm = sqrtf(bcx * bcx + bcy * bcy + bcz * bcz);
-    const float dmin = fmaxf(bnorm - sqrtf(2.f * kHalfEdge * kHalfEdge + hz * hz), 0.f);
-    const float eps = 0.02f + 1e-5f * bnorm;
     for (int base = sBegin; base < sEnd; base += 64) {
-      const int si = base + lane;
-      bool pass = false;
-      if (si < sEnd) {
-        const umpa_traverse_space &sp = a.spaces[si];
-        const float(*t)[3] = sp.transformInv;
-        const float qz = t[2][0] * bcx + t[2][1] * bcy + t[2][2] * bcz;
-        const float ez = kHalfEdge * (fabsf(t[2][0]) + fabsf(t[2][1])) + hz * fabsf(t[2][2]);
-        const float qx = t[0][0] * bcx + t[0][1] * bcy + t[0][2] * bcz;
-        const float ex = kHalfEdge * (fabsf(t[0][0]) + fabsf(t[0][1])) + hz * fabsf(t[0][2]);
-        pass = fabsf(qz) <= a.cullHalfWidth + ez + eps && qx >= -a.cullHalfSpace - ex - eps &&
-               dmin * dmin <= sp.maxDistanceSqr * 1.0001f + eps && sp.projectionIndex >= 0 &&
-               sp.projectionIndex < a.nProj && sp.dir >= 0 && sp.dir <= 2;
-      }
-      unsigned long long mask = __ballot(pass);
+      unsigned long long mask = __ballot(cull_space(a, bk, base + lane, sEnd));
       while (mask) {
         const int k0 = __builtin_ctzll(mask);
         mask &= mask - 1;
@@ -588,22 +669,10 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
           (void *)((const char *)a.fft + off), 0, (int)(a.sliceElems * kPix), kBufferConfig);
         const float scale = kHalf ? a.scales[proj] : 1.f;
-        const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
-                    t22 = sp.transformInv[2][2];
 
         // phase 1: slab test of the column's nz voxels (nz is wave-uniform: shallow bricks
-        // run fewer levels).  qz = (t20 px + t21 py) + t22 pz is the reference's left-to-right
-        // sum, so the shared partial sum changes no bit.
-        const float cz = t20 * px + t21 * py;
-        int cnt = 0;
-        for (int k = 0; k < nz; ++k) {
-          const float pz = (float)(bz + k - cyi);
-          const float qz = cz + t22 * pz;
-          const bool ok = colIn && !(qz * qz > slab2);
-          const unsigned long long m = __ballot(ok);
-          if (ok) ws.queue[cnt + (int)lanes_below(m)] = (uint16_t)(k * 64 + lane);
-          cnt += __popcll(m);
-        }
+        // run fewer levels)
+        const int cnt = slab_queue(ws, a, bk, sp, slab2, lane);
         if (cnt == 0) continue;
         wave_sync();
         const bool unit = sp.weight == 1.f;
@@ -672,24 +741,7 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
         wave_sync();
       }
     }
-    for (int k = 0; k < nz; ++k) {
-      if (colIn) {
-        const int li = k * 64 + lane;
-        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
-        if (split) {
-          float *vol = reinterpret_cast<float *>(a.volume);
-          if (ws.accW[li] != 0.f || ws.accR[li] != 0.f || ws.accI[li] != 0.f) {
-            atomicAdd(vol + 2 * gi, ws.accR[li]);
-            atomicAdd(vol + 2 * gi + 1, ws.accI[li]);
-            atomicAdd(a.weights + gi, ws.accW[li]);
-          }
-        } else {
-          reinterpret_cast<float2 *>(a.volume)[gi] = make_float2(ws.accR[li], ws.accI[li]);
-          a.weights[gi] = ws.accW[li];
-        }
-      }
-    }
-    wave_sync();
+    store_brick(ws, a, bk, lane, split);
     if (heavy) __builtin_amdgcn_s_setprio(0);
     if constexpr (kCount) {
       if (lane == 0 && nupd) atomicAdd(l.updates, nupd);
@@ -779,8 +831,6 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
   const int lane = threadIdx.x & 63;
   WaveScratch &ws = s_wave[threadIdx.x >> 6];
   WaveStage &st = s_stage[threadIdx.x >> 6];
-  const int lx = lane & (kEdge - 1), ly = lane / kEdge;
-  constexpr float kHalfEdge = 0.5f * (kEdge - 1);
   // one resource over the chunk's slices (the host keeps it under 2 GiB); a candidate row
   // read past its slice's end reads the next slice or zeros, always at weight 0
   const uint32_t sliceBytes = (uint32_t)(a.sliceElems * kPix);
@@ -797,74 +847,25 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
     if (b >= l.nItems) break;
     const int4 bo = make_uniform(reinterpret_cast<const int4 *>(l.items)[b]);
     // work item: brick 8 x 8 x depth at (bo.x, bo.y, bo.z); the exact schedules never split
-    const int bz = bo.z & 0xffff;
     const int sBegin = a.spaceBegin, sEnd = a.spaceEnd;
     if (sBegin >= sEnd) continue;
-    const int depth = bo.w & 15;
     const bool heavy = (bo.w >> 4) & 1;
     if (heavy) __builtin_amdgcn_s_setprio(2);
-    const int x = bo.x + lx, y = bo.y + ly;
-    const bool colIn = x < a.vx && y < a.vy;
-    const int nz = min(depth, a.vz - bz);
-    const float px = (float)(x - cxi), py = (float)(y - cyi);
-    for (int k = 0; k < nz; ++k) {
-      const int li = k * 64 + lane;
-      float vr = 0.f, vi = 0.f, vw = 0.f;
-      if (colIn) {
-        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
-        const float2 t = reinterpret_cast<const float2 *>(a.volume)[gi];
-        vr = t.x;
-        vi = t.y;
-        vw = a.weights[gi];
-      }
-      ws.accR[li] = vr;
-      ws.accI[li] = vi;
-      ws.accW[li] = vw;
-    }
-    wave_sync();
-    const float hz = 0.5f * (float)(depth - 1);
-    const float bcx = (float)(bo.x - cxi) + kHalfEdge;
-    const float bcy = (float)(bo.y - cyi) + kHalfEdge;
-    const float bcz = (float)(bz - cyi) + hz;
-    const float bnorm = sqrtf(bcx * bcx + bcy * bcy + bcz * bcz);
-    const float dmin = fmaxf(bnorm - sqrtf(2.f * kHalfEdge * kHalfEdge + hz * hz), 0.f);
-    const float eps = 0.02f + 1e-5f * bnorm;
+    const BrickCtx bk = make_brick(a, bo, lane);
+    const int bz = bk.z0;
+    load_brick(ws, a, bk, lane, false);
     int head = 0, waiting = 0;  // the ring's oldest entry and its number of entries
     uint32_t ordinal = 0;       // spaces taken so far by this work item (mod 2^16)
     for (int base = sBegin; base < sEnd; base += 64) {
-      const int si = base + lane;
-      bool pass = false;
-      if (si < sEnd) {
-        const umpa_traverse_space &sp = a.spaces[si];
-        const float(*t)[3] = sp.transformInv;
-        const float qz = t[2][0] * bcx + t[2][1] * bcy + t[2][2] * bcz;
-        const float ez = kHalfEdge * (fabsf(t[2][0]) + fabsf(t[2][1])) + hz * fabsf(t[2][2]);
-        const float qx = t[0][0] * bcx + t[0][1] * bcy + t[0][2] * bcz;
-        const float ex = kHalfEdge * (fabsf(t[0][0]) + fabsf(t[0][1])) + hz * fabsf(t[0][2]);
-        pass = fabsf(qz) <= a.cullHalfWidth + ez + eps && qx >= -a.cullHalfSpace - ex - eps &&
-               dmin * dmin <= sp.maxDistanceSqr * 1.0001f + eps && sp.projectionIndex >= 0 &&
-               sp.projectionIndex < a.nProj && sp.dir >= 0 && sp.dir <= 2;
-      }
-      unsigned long long mask = __ballot(pass);
+      unsigned long long mask = __ballot(cull_space(a, bk, base + lane, sEnd));
       while (mask) {
         const int k0 = __builtin_ctzll(mask);
         mask &= mask - 1;
         const umpa_traverse_space sp = a.spaces[__builtin_amdgcn_readfirstlane(base + k0)];
         const uint32_t sliceOff =
           (uint32_t)__builtin_amdgcn_readfirstlane(sp.projectionIndex) * sliceBytes;
-        const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
-                    t22 = sp.transformInv[2][2];
         // phase 1: the slab test, as in fr_insert_kernel
-        const float cz = t20 * px + t21 * py;
-        int cnt = 0;
-        for (int k = 0; k < nz; ++k) {
-          const float pz = (float)(bz + k - cyi);
-          const float qz = cz + t22 * pz;
-          const bool ok = colIn && !(qz * qz > r2);
-          const unsigned long long m = __ballot(ok);
-          if (ok) ws.queue[cnt + (int)lanes_below(m)] = (uint16_t)(k * 64 + lane);
-          cnt += __popcll(m);
-        }
+        const int cnt = slab_queue(ws, a, bk, sp, r2, lane);
         if (cnt == 0) continue;
         wave_sync();
         // phase 2: the rest of the predicate; hits go to the ring, gathered 64 at a time
@@ -907,15 +908,7 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
       staged_gather<kNB>(a, nTable, rsrc, tbl, ws, st, lane, head, waiting);
       wave_sync();
     }
-    for (int k = 0; k < nz; ++k) {
-      if (colIn) {
-        const int li = k * 64 + lane;
-        const int64_t gi = ((int64_t)(bz + k) * a.vy + y) * a.vx + x;
-        reinterpret_cast<float2 *>(a.volume)[gi] = make_float2(ws.accR[li], ws.accI[li]);
-        a.weights[gi] = ws.accW[li];
-      }
-    }
-    wave_sync();
+    store_brick(ws, a, bk, lane, false);
     if (heavy) __builtin_amdgcn_s_setprio(0);
   }
 }
