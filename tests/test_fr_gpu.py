@@ -303,3 +303,42 @@ def test_python_mirror_refuses_short_strided_or_host_payloads():
         op.execute(fft.cpu(), vol, wts, sp, tab)
     torch.cuda.synchronize()
     assert float(wts.abs().sum()) == 0.0 and float(vol.abs().sum()) == 0.0
+
+
+def test_two_handles_on_two_streams_run_independently():
+    """Distinct operation handles are independent: two exact-strategy insertions enqueued on
+    two streams at once give the bits each gives alone."""
+    import torch
+    s = settings()
+    dev = torch.device("cuda:0")
+    jobs = []
+    for seed in (21, 22):
+        fft, spaces, desc, table = problem(96, 120, s, seed=seed)
+        jobs.append((torch.from_numpy(np.ascontiguousarray(fft)).to(dev),
+                     torch.from_numpy(np.ascontiguousarray(spaces).view(np.uint8)).to(dev),
+                     desc, torch.from_numpy(table).to(dev)))
+
+    def insert(job, stream):
+        fft, sp, desc, tab = job
+        stream.wait_stream(torch.cuda.current_stream())  # the uploads above
+        op = F.FourierReconstruction(0, stream=stream)
+        assert op.init(desc, s)
+        with torch.cuda.stream(stream):
+            vol = torch.zeros((97,) * 3, dtype=torch.complex64, device=dev)
+            wts = torch.zeros((97,) * 3, dtype=torch.float32, device=dev)
+        assert op.execute(fft, vol, wts, sp, tab), op.last_status
+        return op, vol, wts
+
+    alone = []
+    for job in jobs:
+        st = torch.cuda.Stream()
+        op, vol, wts = insert(job, st)
+        op.synchronize()
+        alone.append((vol.clone(), wts.clone()))
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    both = [insert(job, st) for job, st in zip(jobs, streams)]
+    for (op, vol, wts), (v0, w0) in zip(both, alone):
+        op.synchronize()
+        assert float(w0.max()) > 0
+        assert torch.equal(torch.view_as_real(vol), torch.view_as_real(v0))
+        assert torch.equal(wts, w0)
