@@ -54,6 +54,26 @@ def test_op_autotune_keeps_results_and_persists(tuning_dir):
     assert len((tuning_dir / "fr_tuning.txt").read_text().splitlines()) == 1
 
 
+def test_damaged_cache_entries_are_ignored(tuning_dir):
+    """The last valid line of a key wins; lines with knobs outside their ranges (a damaged or
+    hand-edited file) are skipped."""
+    s = T.settings()
+    box, n = 48, 30
+    _, _, desc, _, d_fft, d_sp, d_tab = _inputs(box, n, s, seed=4)
+    op = F.FourierReconstruction(0)
+    assert op.init(desc, s)
+    p = op.autotune(d_fft, d_sp, d_tab, use_cache=False)
+    path = tuning_dir / "fr_tuning.txt"
+    key = path.read_text().split()[0]
+    with open(path, "a") as f:
+        for bad in ("nan 1 1 0.5", "-2 1 1 0.5", "1 1 999 0.5", "1 1 1 0", "1 1 1 inf", "x y"):
+            f.write(f"{key} {bad} 1.0\n")
+    assert op.autotune(d_fft, d_sp, d_tab, use_cache=True).as_tuple() == p.as_tuple()
+    with open(path, "a") as f:
+        f.write(f"{key} 2 0 1 0.5 0.1\n")
+    assert op.autotune(d_fft, d_sp, d_tab, use_cache=True).as_tuple() == (2.0, 0, 1, 0.5)
+
+
 def test_op_autotune_fast_math_within_tolerance(tuning_dir):
     s = T.settings()
     box, n = 64, 80
