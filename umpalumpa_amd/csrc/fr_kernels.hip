@@ -485,6 +485,26 @@ __device__ __forceinline__ T make_uniform(const T &v)
   return out;
 }
 
+// Element `index` (wave-uniform) of a buffer the kernels only read, through the constant
+// address space: the compiler then reads it with scalar loads into SGPRs instead of 16-byte
+// vector loads of one address.  The exact kernel with cross-space batches reads its space
+// records this way (1.75 -> 1.62 ms per box-256 chunk).  In fr_insert_kernel the same change
+// made the fast-math and FP16 kernels slower (1.46 -> 1.94 and 1.39 -> 1.87 ms; their SGPR
+// count reached 100, with spills through VGPR lanes), so they keep the vector loads.
+template <typename T>
+__device__ __forceinline__ T uniform_load(const T *p, int index)
+{
+  static_assert(sizeof(T) % 4 == 0, "dword-sized records only");
+  typedef __attribute__((address_space(4))) const int const_int;
+  const const_int *src = (const const_int *)(p + index);
+  int w[sizeof(T) / 4];
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = src[i];
+  T out;
+  __builtin_memcpy(&out, w, sizeof(T));
+  return out;
+}
+
 // ---- the brick's steps shared by the insertion kernels ----------------------------------------
 constexpr float kHalfEdge = 0.5f * (kEdge - 1);
 
@@ -845,7 +865,7 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
     if (lane == 0) b = atomicAdd(l.counter, 1);
     b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
     if (b >= l.nItems) break;
-    const int4 bo = make_uniform(reinterpret_cast<const int4 *>(l.items)[b]);
+    const int4 bo = uniform_load(reinterpret_cast<const int4 *>(l.items), b);
     // work item: brick 8 x 8 x depth at (bo.x, bo.y, bo.z); the exact schedules never split
     const int sBegin = a.spaceBegin, sEnd = a.spaceEnd;
     if (sBegin >= sEnd) continue;
@@ -861,7 +881,8 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
       while (mask) {
         const int k0 = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const umpa_traverse_space sp = a.spaces[__builtin_amdgcn_readfirstlane(base + k0)];
+        const umpa_traverse_space sp =
+          uniform_load(a.spaces, __builtin_amdgcn_readfirstlane(base + k0));
         const uint32_t sliceOff =
           (uint32_t)__builtin_amdgcn_readfirstlane(sp.projectionIndex) * sliceBytes;
         // phase 1: the slab test, as in fr_insert_kernel
