@@ -177,7 +177,8 @@ int umpa_fr_batch_create(umpa_fr_batch **b, int device, void *stream);
  * sharing settings, table and slice precision; work items (bricks of all jobs of the step)
  * are ordered heaviest first.  With `useGraph`, the whole loop is captured once as a HIP graph
  * and replayed by each umpa_fr_batch_execute.  The job array is copied; the buffers it names
- * must stay valid while the plan is used. */
+ * must stay valid while the plan is used.  kLookup jobs keep their blob table in LDS: tables
+ * of more than 11,776 samples give UMPA_ERR_UNSUPPORTED. */
 int umpa_fr_batch_plan(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t nJobs,
                        uint32_t chunkSpaces, int strategy, int useGraph);
 

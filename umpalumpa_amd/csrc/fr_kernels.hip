@@ -635,6 +635,7 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
   constexpr int kThreads = 64 * kWaves;
   extern __shared__ float s_table[];
   __shared__ WaveScratch s_wave[kWaves];
+  __shared__ umpa_traverse_space s_space[kWaves];  // the space the wave works on
   const int nTable = l.nTable;
   if constexpr (kMode == kModeLookup && kTableLds) {
     for (int i = threadIdx.x; i < nTable; i += kThreads) s_table[i] = l.table[i];
@@ -681,9 +682,17 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
       while (mask) {
         const int k0 = __builtin_ctzll(mask);
         mask &= mask - 1;
-        // the space index is wave-uniform: say so, so its record is read once per wave and the
-        // slice's buffer descriptor lives in scalar registers
-        const umpa_traverse_space sp = a.spaces[__builtin_amdgcn_readfirstlane(base + k0)];
+        // the space's record goes through the wave's LDS slot: one load of 28 dwords by 28
+        // lanes, then broadcast reads, instead of seven 16-byte loads of one address by every
+        // lane (its 28 SGPRs would not fit beside this kernel's other scalar state)
+        {
+          const int *src =
+            reinterpret_cast<const int *>(a.spaces + __builtin_amdgcn_readfirstlane(base + k0));
+          if (lane < (int)(sizeof(umpa_traverse_space) / 4))
+            reinterpret_cast<int *>(&s_space[threadIdx.x >> 6])[lane] = src[lane];
+          wave_sync();
+        }
+        const umpa_traverse_space &sp = s_space[threadIdx.x >> 6];
         const int proj = __builtin_amdgcn_readfirstlane(sp.projectionIndex);
         const int64_t off = (int64_t)proj * a.sliceElems * kPix;
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(

@@ -44,7 +44,7 @@ struct FrLaunch {
 };
 
 // Largest blob table kept in LDS (floats); batches require it there.
-constexpr int kFrMaxLdsTable = 12000;
+constexpr int kFrMaxLdsTable = 11776;
 
 enum : int { kModeLookup = 0, kModeFastKaiser = 1, kModePreciseKaiser = 2, kModeNearest = 3 };
 
