@@ -342,3 +342,20 @@ def test_two_handles_on_two_streams_run_independently():
         assert float(w0.max()) > 0
         assert torch.equal(torch.view_as_real(vol), torch.view_as_real(v0))
         assert torch.equal(wts, w0)
+
+
+@pytest.mark.parametrize("n_table", [10239, 10240, 11776, 11777, 20000])
+def test_table_sizes_across_kernel_limits(n_table):
+    """Exact strategy with blob tables on both sides of each limit: up to 10,239 samples the
+    kernel with cross-space batches, up to 11,776 the table in LDS beside 16 wavefronts' state,
+    beyond that the table read from global memory.  Bit-identical to the oracle with the same
+    table every time."""
+    s = settings()
+    box, n = 48, 40
+    fft, spaces, _, _ = problem(box, n, s, seed=31)
+    desc = F.fr_desc(box // 2 + 1, box, n, box + 1, len(spaces), n_table)
+    table = F.fill_blob_table(n_table, desc, s)
+    ov, ow, _ = run_oracle(s, fft, spaces, table, box)
+    v, w, _ = run_hip(s, fft, spaces, desc, table, box)
+    assert w.max() > 0
+    assert _bits_equal(v, ov) and _bits_equal(w, ow)
