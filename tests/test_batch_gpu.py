@@ -166,3 +166,20 @@ def test_batch_errors():
     assert e.value.status == F._lib.UMPA_ERR_REJECTED
     with pytest.raises(F._lib.UmpaError):
         b.step(jobs, [0], [jobs[0].n_spaces + 1], F.FR_EXACT)   # range past the spaces
+
+
+def test_batch_refuses_tables_beyond_lds():
+    """Batches keep the blob table in LDS: 11,776 samples plan, 11,777 are unsupported."""
+    s = T.settings()
+    host, jobs = make_jobs(SPECS[:1], s)
+    dev = torch.device("cuda:0")
+    b = F.FRBatch(0)
+    for n_table, ok in ((11776, True), (11777, False)):
+        tab = torch.zeros(n_table, dtype=torch.float32, device=dev)
+        j = F.BatchJob(jobs[0].fft, jobs[0].volume, jobs[0].weights, jobs[0].spaces, s, tab)
+        if ok:
+            b.plan([j], 8, F.FR_EXACT)
+        else:
+            with pytest.raises(F._lib.UmpaError) as e:
+                b.plan([j], 8, F.FR_EXACT)
+            assert e.value.status == F._lib.UMPA_ERR_UNSUPPORTED
