@@ -3,10 +3,11 @@
 // The reference scatters each rotated slice into the volume with three float atomics per voxel
 // (fr_cuda_kernels.cu:266-365).  Here a wavefront owns a brick of 8 x 8 x D voxels (a z column
 // of D per lane, D = 8, 4, 2 or 1 by distance from the centre) for the whole work item and
-// keeps their running (complex, weight) sums in its LDS slot.  It culls the traverse spaces against the brick 64 at a time with one ballot and walks
-// the surviving spaces in index order, so every voxel receives its per-space contributions in
-// the order of the reference CPU loop over spaces (fr_cpu_kernels.hpp:254): no atomics, and no
-// dependence on how bricks are scheduled.
+// keeps their running (complex, weight) sums in its LDS slot.  It culls the traverse spaces
+// against the brick 64 at a time with one ballot and walks the surviving spaces in index
+// order, so every voxel receives its per-space contributions in the order of the reference CPU
+// loop over spaces (fr_cpu_kernels.hpp:254): no atomics, and no dependence on how bricks are
+// scheduled.
 //
 // Per (brick, space) pair the wave
 //   1. tests the reference's slab condition (q.z^2 <= r^2) for its voxels, 4 VALU each, and
@@ -17,8 +18,12 @@
 //      slice values), so the sums are bit-identical to the reference's skip-the-pixel loop
 //      while all table and pixel loads of an item issue back to back; each voxel appears once
 //      per space, so its lane adds the contribution to the voxel's LDS sum directly.
-// Workgroups are persistent (one per CU: 16 or 12 wavefronts sharing the blob table in LDS) and take
-// bricks from a counter, most central (heaviest) first.
+// Workgroups are persistent (one per CU: 16 or 12 wavefronts sharing the blob table in LDS)
+// and take bricks from a counter, most central (heaviest) first.
+//
+// fr_insert_kernel covers every mode, precision and strategy; fr_insert_staged_kernel, for the
+// exact FP32 lookup strategy, gathers the predicate's hits of several spaces together (see
+// there).
 #include "fr_device.hpp"
 
 namespace umpa {
