@@ -8,7 +8,8 @@ in double), stated per workload on sub-problems the oracle finishes in seconds:
 
 FP16 errors are measured against the FP64 oracle on the dequantized slices (the precision of
 the accumulation); the quantization itself is stated separately in the FP16 test of
-test_fr_gpu.py.  Bounds: 1e-4 for every FP16 path and for FP32."""
+test_fr_gpu.py.  Bounds: 1e-4 relative L2 for every FP16 path (the stated FP16 tolerance
+against FP64) and for FP32."""
 import numpy as np
 import pytest
 
