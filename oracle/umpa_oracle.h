@@ -4,7 +4,8 @@
  * CPU restatement of Umpalumpa's Fourier-reconstruction (FR) insertion path and of the
  * Fourier-processing (FP) crop step that feeds it.  It exists to check the HIP product path:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only as
- * the checker or the reported CPU baseline, never as a fallback of the product.
+ * the reference the HIP results are compared with or the reported CPU baseline, never as a
+ * fallback of the product.
  *
  * Every function names the reference file:line it follows (paths relative to the reference
  * repository root).  Parity pins, per function:

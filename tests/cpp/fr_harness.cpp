@@ -6,7 +6,7 @@
 // variant of the harness.  The last batch holds the remaining images (fewer than at Init).
 //
 // Usage: fr_harness OUTDIR [X Y N BATCH GROUP]   (defaults 32 32 20 8 C2)
-// Writes the inputs and results as raw little-endian arrays to OUTDIR for the checker
+// Writes the inputs and results as raw little-endian arrays to OUTDIR for the comparison
 // (tests/test_cpp_api.py): meta.txt, images.f32 [N][Y][X], spaces.bin (112-byte records, each
 // batch's projection indices relative to the batch), crops.c64 [N][Y][X/2] (the FP output the
 // insertion read), volume.c64 and weights.f32 [Y+1][Y+1][X+1].

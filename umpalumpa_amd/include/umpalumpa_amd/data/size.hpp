@@ -1,5 +1,5 @@
 // Size of up to three dimensions [x, y, z] plus a count n: the semantics of
-// umpalumpa::data::Size (data/size.hpp:23-103) in this build's own code.
+// umpalumpa::data::Size (data/size.hpp:23-103), written anew for this library.
 #pragma once
 
 #include <cstddef>
