@@ -608,9 +608,13 @@ __device__ __forceinline__ bool cull_space(const FrJob &a, const BrickCtx &b, in
   const float ez = kHalfEdge * (fabsf(t[2][0]) + fabsf(t[2][1])) + b.hz * fabsf(t[2][2]);
   const float qx = t[0][0] * b.bcx + t[0][1] * b.bcy + t[0][2] * b.bcz;
   const float ex = kHalfEdge * (fabsf(t[0][0]) + fabsf(t[0][1])) + b.hz * fabsf(t[0][2]);
-  return fabsf(qz) <= a.cullHalfWidth + ez + b.eps && qx >= -a.cullHalfSpace - ex - b.eps &&
-         b.dmin * b.dmin <= sp.maxDistanceSqr * 1.0001f + b.eps && sp.projectionIndex >= 0 &&
-         sp.projectionIndex < a.nProj && sp.dir >= 0 && sp.dir <= 2;
+  // every condition is evaluated (&, not &&), so the record's fields load together instead of
+  // one after another behind the earlier tests
+  const float mds = sp.maxDistanceSqr;
+  const int proj = sp.projectionIndex, dir = sp.dir;
+  return (fabsf(qz) <= a.cullHalfWidth + ez + b.eps) & (qx >= -a.cullHalfSpace - ex - b.eps) &
+         (b.dmin * b.dmin <= mds * 1.0001f + b.eps) & (proj >= 0) & (proj < a.nProj) &
+         (dir >= 0) & (dir <= 2);
 }
 
 // Phase 1: the slab test q.z^2 <= slab2 of the column's nz voxels; the voxels that pass go to
