@@ -446,15 +446,16 @@ __device__ __forceinline__ bool blob_predicate(const umpa_traverse_space &s, con
     qz = t[2][0] * px + t[2][1] * py + t[2][2] * pz;
   }
   zSqr = qz * qz;
-  bool pass = !(dsq > s.maxDistanceSqr) && !(qx < -r) && !(zSqr > r * r);
+  // & rather than &&: every test is evaluated, so the record's fields load together
+  bool pass = !(dsq > s.maxDistanceSqr) & !(qx < -r) & !(zSqr > r * r);
   qy += (float)(c.cMaxVolumeIndexYZ / 2);
   // traversal: the column must lie in the AABB's plane orthogonal to `dir`
   if (s.dir == 0) {
-    pass = pass && x >= s.minX && x <= s.maxX && y >= s.minY && y <= s.maxY;
+    pass = pass & (x >= s.minX) & (x <= s.maxX) & (y >= s.minY) & (y <= s.maxY);
   } else if (s.dir == 1) {
-    pass = pass && x >= s.minX && x <= s.maxX && z >= s.minZ && z <= s.maxZ;
+    pass = pass & (x >= s.minX) & (x <= s.maxX) & (z >= s.minZ) & (z <= s.maxZ);
   } else {
-    pass = pass && y >= s.minY && y <= s.maxY && z >= s.minZ && z <= s.maxZ;
+    pass = pass & (y >= s.minY) & (y <= s.maxY) & (z >= s.minZ) & (z <= s.maxZ);
   }
   // ... and the voxel must lie in the column's slab between the two bounding planes.  A voxel
   // whose |qz| is at least 0.001 inside the blob radius is between the planes by at least that
