@@ -692,19 +692,23 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
     unsigned long long nupd = 0;
     for (int base = sBegin; base < sEnd; base += 64) {
       unsigned long long mask = __ballot(cull_space(a, bk, base + lane, sEnd));
+      // the space's record goes through the wave's LDS slot: one load of 28 dwords by 28
+      // lanes, then broadcast reads, instead of seven 16-byte loads of one address by every
+      // lane (its 28 SGPRs would not fit beside this kernel's other scalar state). The next
+      // surviving space's dword is loaded while the current space is processed.
+      constexpr int kRecDwords = (int)(sizeof(umpa_traverse_space) / 4);
+      const bool recLane = lane < kRecDwords;
+      int rec = 0;
+      if (mask && recLane)
+        rec = reinterpret_cast<const int *>(
+          a.spaces + __builtin_amdgcn_readfirstlane(base + __builtin_ctzll(mask)))[lane];
       while (mask) {
-        const int k0 = __builtin_ctzll(mask);
         mask &= mask - 1;
-        // the space's record goes through the wave's LDS slot: one load of 28 dwords by 28
-        // lanes, then broadcast reads, instead of seven 16-byte loads of one address by every
-        // lane (its 28 SGPRs would not fit beside this kernel's other scalar state)
-        {
-          const int *src =
-            reinterpret_cast<const int *>(a.spaces + __builtin_amdgcn_readfirstlane(base + k0));
-          if (lane < (int)(sizeof(umpa_traverse_space) / 4))
-            reinterpret_cast<int *>(&s_space[threadIdx.x >> 6])[lane] = src[lane];
-          wave_sync();
-        }
+        if (recLane) reinterpret_cast<int *>(&s_space[threadIdx.x >> 6])[lane] = rec;
+        wave_sync();
+        if (mask && recLane)
+          rec = reinterpret_cast<const int *>(
+            a.spaces + __builtin_amdgcn_readfirstlane(base + __builtin_ctzll(mask)))[lane];
         const umpa_traverse_space &sp = s_space[threadIdx.x >> 6];
         const int proj = __builtin_amdgcn_readfirstlane(sp.projectionIndex);
         const int64_t off = (int64_t)proj * a.sliceElems * kPix;
