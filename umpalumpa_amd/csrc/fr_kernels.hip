@@ -602,7 +602,7 @@ __device__ __forceinline__ void store_brick(const WaveScratch &ws, const FrJob &
 // with the brick's extents as margins; spaces naming a slice that does not exist never pass.
 // Bricks this far from the centre (in voxels) exist only in volumes above box 256; there most
 // spaces miss the brick's slab, so the cull rejects on the slab test before it loads the rest of
-// the record (box 1024, D7: 141.5 -> 130.6 ms per 512-slice step), while closer bricks load every
+// the record (box 1024, D7: 141.5 -> 131.2 ms per 512-slice step), while closer bricks load every
 // field at once (box 256: the early rejection would cost 1 to 3 %).
 constexpr float kEarlyCullRadius = 128.f;
 template <bool kEarly>
