@@ -154,7 +154,11 @@ int umpa_fr_last_updates(umpa_fr_op *op, uint64_t *updates);
 /* One reconstruction of a batch.  desc/settings as for umpa_fr_init (desc.fft.n = slices in
  * `fft`); spaces: desc.traverseSpaces records, projectionIndex < desc.fft.n; fft complex
  * float, or complex half when `scales` (one float per slice) is not NULL; all pointers device
- * memory; table: device blob table of desc.blobTable floats (kLookup). */
+ * memory; table: device blob table of desc.blobTable floats (kLookup).  The volumes and
+ * weights of the jobs passed to one plan / step must not overlap one another (each job owns
+ * its outputs; a voxel is summed by one wavefront without atomics): overlapping outputs give
+ * UMPA_ERR_INVALID_ARG.  To insert two slice sets into one volume, make them one job or run
+ * them in successive steps. */
 typedef struct umpa_fr_job {
   umpa_fr_desc desc;
   umpa_fr_settings settings;
@@ -191,6 +195,11 @@ int umpa_fr_batch_execute(umpa_fr_batch *b);
 int umpa_fr_batch_step(umpa_fr_batch *b, const umpa_fr_job *jobs, const uint64_t *begin,
                        const uint64_t *end, uint32_t nJobs, int strategy);
 
+/* The checks umpa_fr_batch_plan and umpa_fr_batch_step apply to a job list (each job as
+ * umpa_fr_init + umpa_fr_execute would check it, and outputs that do not overlap), on the
+ * host: no device memory is read and no device is needed.  Same status codes. */
+int umpa_fr_batch_check_jobs(const umpa_fr_job *jobs, uint32_t nJobs, int strategy);
+
 /* number of steps of the current plan */
 int umpa_fr_batch_num_steps(umpa_fr_batch *b, uint32_t *steps);
 /* counting variant for the next execute / step (bypasses the graph); result after it */
@@ -223,6 +232,9 @@ int umpa_fr_autotune(umpa_fr_op *op, const void *fft, const float *scales, uint6
                      const umpa_traverse_space *spaces, uint64_t nSpaces, const float *table,
                      int useCache, umpa_fr_sched_params *chosen);
 int umpa_fr_get_sched_params(umpa_fr_op *op, umpa_fr_sched_params *out);
+/* Sets the schedule knobs the op uses under `strategy` (its work items are rebuilt; waits for
+ * the op's stream first).  UMPA_FR_EXACT ignores maxParts / splitAt (it never splits). */
+int umpa_fr_set_sched_params(umpa_fr_op *op, int strategy, const umpa_fr_sched_params *p);
 
 /* Tunes the schedule of a batch's later plans / steps under `strategy` on the first step of
  * the chunk loop these jobs would form. */
@@ -233,6 +245,16 @@ int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy,
                                    const umpa_fr_sched_params *p);
 
 /* ---- FR host-side helpers (no GPU needed) ----------------------------------------------- */
+
+/* The insertion's work items for a vx * vy * vz volume under the schedule knobs `p`, heaviest
+ * first, as int32 quadruples {x, y, z, flags}: the brick at (x, y, z) is 8 x 8 x D voxels with
+ * D = flags & 15, raised priority when flags bit 4 is set, and a brick split over P = (flags
+ * >> 8) & 255 wavefronts (P > 1 only with splitHeavy, the fast-math schedules) appears P times
+ * with part (flags >> 16) & 255.  *count receives the number of items; items may be NULL to
+ * query it, else capacity must hold them. */
+int umpa_fr_brick_schedule(uint32_t vx, uint32_t vy, uint32_t vz, int splitHeavy,
+                           const umpa_fr_sched_params *p, int32_t *items, uint64_t capacity,
+                           uint64_t *count);
 
 /* AFR::CreateConstants (afr.cpp:6-19) */
 int umpa_fr_create_constants(const umpa_fr_desc *desc, const umpa_fr_settings *settings,

@@ -14,15 +14,22 @@
  *   orc_kaiser_fourier_value,
  *   orc_create_constants,
  *   orc_fill_blob_table         closed forms (I0(15), I_{3/2}(15), KB Fourier value at 0) to
- *                               1e-6 relative, table monotonicity and end points (the reference
- *                               afr.cpp / kaiser.hpp need spdlog, absent here: not compiled)
+ *                               1e-6 relative, table monotonicity and end points; orders 1-4
+ *                               against s^k I_k(alpha s) / I_k(alpha) (mpmath), and the NaN
+ *                               cIw0 / table that the order-0-only Fourier value gives them (the
+ *                               reference afr.cpp / kaiser.hpp need spdlog, absent here: not
+ *                               compiled)
  *   orc_compute_traverse_space  reference KATs (afr_common.hpp:136-254) + bit-exact vs the
  *                               compiled reference traverse_space_generator.hpp
  *   orc_fr_execute              bit-exact vs the compiled reference fr_cpu_kernels.hpp golden
- *                               vectors (tests/golden), zero/non-zero pattern of the reference
- *                               tests, brute-force per-voxel gather on tiny inputs
- *   orc_fr_execute_f64          agreement with orc_fr_execute to FP32 accumulation error and
- *                               exact agreement on integer-valued inputs (see tests)
+ *                               vectors (tests/golden; every mode, blob orders 0-4), zero/non-
+ *                               zero pattern of the reference tests, brute-force per-voxel
+ *                               gather on tiny inputs
+ *   orc_fr_execute_f64          exact agreement with orc_fr_execute where FP32 is exact (axis-
+ *                               aligned rotations, integer slices, table and space weights),
+ *                               agreement to FP32 accumulation error with random orientations
+ *                               and non-unit space weights, exact scaling by a power-of-two
+ *                               weight (tests/test_oracle.py)
  *   orc_fp_execute              bit-exact vs the compiled reference fp_cpu_kernels.hpp and the
  *                               reference value tests (fp_tests.hpp / afp_common.hpp:132-178)
  */

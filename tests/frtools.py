@@ -60,14 +60,17 @@ def run_oracle_f64(s, fft, spaces, table, box):
 
 
 def run_hip(s, fft, spaces, desc, table, box, strategy=F.FR_EXACT, chunks=1, count=False,
-            half=None, volume=None, weights=None):
+            half=None, volume=None, weights=None, sched=None):
     """Insert on cuda:0; `chunks` splits the spaces over several executes.  `half` is a pair
-    (float16 [n, hy, hx, 2] array, float32 scales) for the FP16 path."""
+    (float16 [n, hy, hx, 2] array, float32 scales) for the FP16 path; `sched` forces the
+    schedule knobs (a SchedParams) of the strategy."""
     import torch
     dev = torch.device("cuda:0")
     op = F.FourierReconstruction(0)
     assert op.init(desc, s), op.last_status
     op.set_strategy(strategy)
+    if sched is not None:
+        op.set_sched_params(strategy, sched)
     op.set_count_updates(count)
     if volume is None:
         vol = torch.zeros((box + 1,) * 3, dtype=torch.complex64, device=dev)
@@ -103,6 +106,17 @@ def quantize_f16(fft):
     h = (ri / s[:, None, None, None]).astype(np.float16)
     deq = (h.astype(np.float32) * s[:, None, None, None]).astype(np.float32)
     return h, s.astype(np.float32), (deq[..., 0] + 1j * deq[..., 1]).astype(np.complex64)
+
+
+def rel_l2_big(a, b, rows=16):
+    """rel_l2 over large volumes, a few z planes at a time (no full-size temporaries)."""
+    num = den = 0.0
+    for z in range(0, a.shape[0], rows):
+        x = np.asarray(a[z:z + rows], dtype=np.complex128)
+        y = np.asarray(b[z:z + rows], dtype=np.complex128)
+        num += float(np.sum(np.abs(x - y) ** 2))
+        den += float(np.sum(np.abs(y) ** 2))
+    return (num / den) ** 0.5 if den > 0 else num ** 0.5
 
 
 def rel_l2(a, b):

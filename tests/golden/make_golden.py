@@ -12,6 +12,9 @@ Each fixture stores inputs and the reference outputs (no reference source):
   fr_box32.npz       config-1 value distribution at a small box: 40 slices 17x32, 33^3 volume
   fp_ref_cases.npz   the 9 FP cases of fourier_processing/fp_tests.hpp
   ts_random.npz      traverse spaces of random rotations
+  fr_orders.npz      blob orders 1-4 (and 0) of every blob mode, dispatched on the order by
+                     the reference's ExpandBools (fr_cpu.cpp:34-43); the order-0-only
+                     KaiserFourierValue (kaiser.hpp:66-81) makes cIw0 NaN for orders 1-4
 """
 from __future__ import annotations
 
@@ -110,6 +113,38 @@ def random_fr(R):
     return out
 
 
+def orders_fr(R):
+    """Blob orders 0-4 in every blob mode: kDynamic with alpha 16 (precise Kaiser,
+    fr_common_kernels.hpp:129-161), kDynamic with alpha 15 (kaiserValueFast), kLookup (table
+    from FillBlobTable's arithmetic), and the nearest-pixel traversal (order ignored)."""
+    out = {}
+    O = P.Oracle()
+    modes = [("precise", P.K_DYNAMIC, P.K_PRECISE, 16.0),
+             ("fast", P.K_DYNAMIC, P.K_PRECISE, 15.0),
+             ("lookup", P.K_LOOKUP, P.K_PRECISE, 15.0),
+             ("nearest", P.K_DYNAMIC, P.K_FAST, 16.0)]
+    for k, (mname, interp, typ, alpha) in enumerate(modes):
+        for order in range(5):
+            s = P.Settings(interp, order, typ, alpha, 1.9)
+            Pb, n = 16, 8
+            rng = np.random.default_rng(300 + 10 * k + order)
+            fft = (rng.standard_normal((n, Pb, Pb // 2 + 1))
+                   + 1j * rng.standard_normal((n, Pb, Pb // 2 + 1))).astype(np.complex64)
+            rots = P.uniform_rotations(n, 400 + 10 * k + order).astype(np.float32)
+            table = O.fill_blob_table(10000, Pb + 1, Pb + 1, s) if interp == P.K_LOOKUP \
+                else None
+            sp, vol, w = fr_case(R, s, Pb, fft, rots, typ == P.K_FAST, table)
+            name = f"{mname}_o{order}"
+            out[name + "/fft"] = fft
+            out[name + "/spaces"] = sp
+            out[name + "/table"] = np.zeros(0, np.float32) if table is None else table
+            out[name + "/volume"] = vol
+            out[name + "/weights"] = w
+            out[name + "/settings"] = np.array([s.interpolation, s.order, s.type, s.alpha,
+                                                s.blobRadius], np.float64)
+    return out
+
+
 def box32(R):
     """Config-1 value distribution (complex Gaussian times 1/(1+k^2)) at P = 32."""
     O = P.Oracle()
@@ -185,6 +220,7 @@ def main():
     np.savez_compressed(os.path.join(HERE, "fr_box32.npz"), **box32(R))
     np.savez_compressed(os.path.join(HERE, "fp_ref_cases.npz"), **reference_fp_cases(R))
     np.savez_compressed(os.path.join(HERE, "ts_random.npz"), **random_ts(R))
+    np.savez_compressed(os.path.join(HERE, "fr_orders.npz"), **orders_fr(R))
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -70,6 +70,49 @@ def test_precise_kaiser_alpha16(golden):
     assert np.array_equal(w != 0, g[mode + "/weights"] != 0)
 
 
+def _same_with_nans(a, b):
+    """Equal NaN positions and bit-identical values elsewhere (the NaN payload is not part of
+    the comparison)."""
+    a = np.asarray(a).view(np.float32)
+    b = np.asarray(b).view(np.float32)
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.where(na, 0, a).tobytes() == np.where(nb, 0, b).tobytes()
+
+
+ORDER_CASES = [f"{m}_o{k}" for m in ("precise", "fast", "lookup", "nearest") for k in range(5)]
+
+
+@pytest.mark.parametrize("case", ORDER_CASES)
+def test_blob_orders_match_reference(case, golden):
+    """Blob orders 0-4 (fr_common_kernels.hpp:129-161) against the compiled reference, with
+    the counting variant on and off (it must evaluate the same order).  Orders 1-4 make cIw0
+    NaN (the order-0-only KaiserFourierValue, kaiser.hpp:73-80), so every voxel a blob reaches
+    is NaN in the reference: the NaN positions must match and every other value bit for bit.
+    Order 0 of the precise Kaiser (expf in the float Bessel) is held to 1e-6 as elsewhere."""
+    g = golden("fr_orders.npz")
+    interp, order, typ, alpha, radius = g[case + "/settings"]
+    s = settings(int(interp), int(typ), float(alpha), float(radius), int(order))
+    table = g[case + "/table"]
+    table = table if table.size else None
+    fft = g[case + "/fft"]
+    box = fft.shape[1]
+    sp = g[case + "/spaces"].view(F.TRAVERSE_SPACE_DTYPE)
+    desc = F.fr_desc(box // 2 + 1, box, fft.shape[0], box + 1, len(sp),
+                     0 if table is None else table.size)
+    gv, gw = g[case + "/volume"], g[case + "/weights"]
+    v0, w0, _ = run_hip(s, fft, sp, desc, table, box)
+    v1, w1, upd = run_hip(s, fft, sp, desc, table, box, count=True)
+    assert _same_with_nans(v0, v1) and _same_with_nans(w0, w1), "counting variant differs"
+    assert upd == run_oracle(s, fft, sp, table, box)[2]
+    if case == "precise_o0":
+        assert rel_l2(v0, gv) <= 1e-6 and rel_l2(w0, gw) <= 1e-6
+        assert np.array_equal(w0 != 0, gw != 0)
+    else:
+        assert _same_with_nans(v0, gv) and _same_with_nans(w0, gw), case
+    if int(order) and not case.startswith("nearest"):
+        assert np.isnan(w0).any() and np.array_equal(np.isnan(w0), gw != 0)
+
+
 def test_box32_golden_bit_exact(golden):
     g = golden("fr_box32.npz")
     s = settings()

@@ -136,6 +136,40 @@ def test_oracle_bessel_bit_exact_vs_reference(oracle, reference):
             assert np.float64(f(x)).tobytes() == np.float64(g(x)).tobytes()
 
 
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("alpha", [15.0, 16.0])
+def test_kaiser_value_orders_closed_form(oracle, order, alpha):
+    """KaiserValue for blob orders 1-4 (kaiser.hpp:30-48) against its closed form
+    s^k I_k(alpha s) / I_k(alpha), s = sqrt(1 - (d/r)^2), with mpmath's Bessel I_k.  The
+    reference evaluates I_2..I_4 by the upward recurrence from polynomial I_0, I_1
+    (bessier.hpp:77-84), which loses relative precision for small arguments: the relative
+    bound holds up to d = 0.95 r, an absolute bound (against a peak value of 1) everywhere."""
+    import mpmath as mp
+    r = 1.9
+    for i in list(range(0, 10000, 41)) + [9990, 9998, 9999]:
+        d = r * math.sqrt(i / 9999)
+        sv = mp.sqrt(1 - (mp.mpf(d) / r) ** 2)
+        want = float(sv ** order * mp.besseli(order, alpha * sv) / mp.besseli(order, alpha))
+        got = oracle.lib.orc_kaiser_value(d, r, alpha, order)
+        assert abs(got - want) <= 1e-6, (order, i, got, want)
+        if d <= 0.95 * r:
+            assert abs(got - want) <= 1e-5 * want, (order, i, got, want)
+    assert oracle.lib.orc_kaiser_value(r * 1.001, r, alpha, order) == 0.0
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_orders_make_constants_and_table_nan(oracle, order):
+    """KaiserFourierValue exists for order 0 only and returns NaN otherwise (kaiser.hpp:
+    73-80), so cIw0 = 1 / NaN and every table entry KaiserValue * cIw0 is NaN (afr.cpp:14,
+    22-34); 1 / I_order(alpha) stays finite."""
+    s = P.Settings(P.K_LOOKUP, order, P.K_PRECISE, 15.0, 1.9)
+    c = oracle.create_constants(33, 33, 10000, s)
+    assert math.isnan(c.cIw0)
+    assert math.isfinite(c.cOneOverBessiOrderAlpha) and c.cOneOverBessiOrderAlpha > 0
+    t = oracle.fill_blob_table(10000, 33, 33, s)
+    assert np.isnan(t).all()
+
+
 # ---- FR insertion: golden vectors of the compiled reference --------------------------------
 def _fr_golden_case(oracle, g, key, fft, spaces, table, box):
     interp, order, typ, alpha, radius = g[key + "/settings"] if key else (1, 0, 1, 15.0, 1.9)
@@ -168,6 +202,28 @@ def test_fr_golden_random_modes(oracle, golden):
                                table if table.size else None, fft.shape[1])
         assert v.tobytes() == g[key + "/volume"].tobytes(), key
         assert w.tobytes() == g[key + "/weights"].tobytes(), key
+
+
+def test_fr_golden_orders(oracle, golden):
+    """Blob orders 0-4 in every blob mode and the nearest-pixel traversal, bit for bit (NaN
+    bits included) against the compiled reference: for orders 1-4 every voxel a blob reaches
+    is NaN (cIw0 is NaN), the nearest-pixel traversal ignores the order."""
+    g = golden("fr_orders.npz")
+    keys = sorted({k.split("/")[0] for k in g.files})
+    assert len(keys) == 20
+    for key in keys:
+        table = g[key + "/table"]
+        fft = g[key + "/fft"]
+        v, w = _fr_golden_case(oracle, g, key, fft, g[key + "/spaces"],
+                               table if table.size else None, fft.shape[1])
+        assert v.tobytes() == g[key + "/volume"].tobytes(), key
+        assert w.tobytes() == g[key + "/weights"].tobytes(), key
+        gw = g[key + "/weights"]
+        order = int(g[key + "/settings"][1])
+        if key.startswith("nearest") or order == 0:
+            assert not np.isnan(gw).any() and (gw != 0).any(), key
+        else:
+            assert (np.isnan(gw) == (gw != 0)).all() and np.isnan(gw).any(), key
 
 
 def test_fr_golden_box32(oracle, golden):
@@ -317,6 +373,93 @@ def test_fr_linearity_and_f64(oracle):
     oracle.fr_execute_f64(s, dv, dw, a, sp, table, c)
     assert np.linalg.norm(va - dv) / np.linalg.norm(dv) < 1e-4
     assert np.linalg.norm(wa - dw) / np.linalg.norm(dw) < 1e-4
+
+
+def _axis_rotations():
+    """Rotations whose entries are 0 and +-1: with them q = T^T p is exact in any precision."""
+    mats = []
+    for perm in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            m = np.zeros((3, 3), np.float32)
+            for i, j in enumerate(perm):
+                m[i, j] = signs[i]
+            if round(float(np.linalg.det(m))) == 1:
+                mats.append(m)
+    return mats
+
+
+def test_fr_f64_exact_on_integer_inputs(oracle):
+    """On inputs where FP32 arithmetic is exact -- axis-aligned rotations (q = T^T p exactly),
+    integer slice values, an integer-valued table and integer space weights -- the FP64
+    oracle must equal the FP32 one exactly.  The weights 1, 2 and 3 of the spaces make a
+    dropped or misplaced space weight visible, the non-constant table a wrong table index,
+    and the 24 rotations a transposed operand."""
+    s = P.Settings(P.K_LOOKUP, 0, P.K_PRECISE, 15.0, 1.9)
+    box = 16
+    rots = _axis_rotations()
+    assert len(rots) == 24
+    n = len(rots)
+    rng = np.random.default_rng(8)
+    hx, hy = box // 2 + 1, box
+    fft = (rng.integers(-4, 5, (n, hy, hx)) + 1j * rng.integers(-4, 5, (n, hy, hx))).astype(
+        np.complex64)
+    sp = np.zeros(n, dtype=P.TS_DTYPE)
+    for i, m in enumerate(rots):
+        sp[i] = oracle.compute_traverse_space(box // 2, box, m, box, box, False, 1.9,
+                                              float(1 + i % 3))
+        sp[i]["projectionIndex"] = i
+    table = (np.arange(10000) % 5 + 1).astype(np.float32)
+    c = oracle.create_constants(box + 1, box + 1, 10000, s)
+    v = np.zeros((box + 1,) * 3, np.complex64)
+    w = np.zeros((box + 1,) * 3, np.float32)
+    oracle.fr_execute(s, v, w, fft, sp, table, c)
+    dv = np.zeros((box + 1,) * 3, np.complex128)
+    dw = np.zeros((box + 1,) * 3, np.float64)
+    oracle.fr_execute_f64(s, dv, dw, fft, sp, table, c)
+    assert (w != 0).sum() > 500
+    assert np.array_equal(dv, v.astype(np.complex128))
+    assert np.array_equal(dw, w.astype(np.float64))
+    # the same records with all weights 1 give a different volume: the weight is used
+    sp1 = sp.copy()
+    sp1["weight"] = 1.0
+    dw1 = np.zeros_like(dw)
+    oracle.fr_execute_f64(s, np.zeros_like(dv), dw1, fft, sp1, table, c)
+    assert not np.array_equal(dw1, dw)
+
+
+def test_fr_f64_non_unit_weights(oracle):
+    """Random orientations with space weights drawn from [0.25, 4]: the FP64 oracle agrees
+    with the FP32 one to FP32 accumulation error, and scaling every weight by 2 scales the
+    FP64 volume and weights by exactly 2 (a power of two changes no rounding)."""
+    s = P.Settings(P.K_LOOKUP, 0, P.K_PRECISE, 15.0, 1.9)
+    box, n = 24, 20
+    rng = np.random.default_rng(12)
+    fft = (rng.standard_normal((n, box, box // 2 + 1)) +
+           1j * rng.standard_normal((n, box, box // 2 + 1))).astype(np.complex64)
+    sp = np.zeros(n, dtype=P.TS_DTYPE)
+    wts = rng.uniform(0.25, 4.0, n).astype(np.float32)
+    for i, m in enumerate(P.uniform_rotations(n, 13).astype(np.float32)):
+        sp[i] = oracle.compute_traverse_space(box // 2, box, m, box, box, False, 1.9,
+                                              float(wts[i]))
+        sp[i]["projectionIndex"] = i
+    table = oracle.fill_blob_table(10000, box + 1, box + 1, s)
+    c = oracle.create_constants(box + 1, box + 1, 10000, s)
+
+    def f64(spaces):
+        dv = np.zeros((box + 1,) * 3, np.complex128)
+        dw = np.zeros((box + 1,) * 3, np.float64)
+        oracle.fr_execute_f64(s, dv, dw, fft, spaces, table, c)
+        return dv, dw
+    v = np.zeros((box + 1,) * 3, np.complex64)
+    w = np.zeros((box + 1,) * 3, np.float32)
+    oracle.fr_execute(s, v, w, fft, sp, table, c)
+    dv, dw = f64(sp)
+    assert np.linalg.norm(v - dv) / np.linalg.norm(dv) < 1e-4
+    assert np.linalg.norm(w - dw) / np.linalg.norm(dw) < 1e-4
+    sp2 = sp.copy()
+    sp2["weight"] = sp["weight"] * 2
+    dv2, dw2 = f64(sp2)
+    assert np.array_equal(dv2, 2 * dv) and np.array_equal(dw2, 2 * dw)
 
 
 def test_fr_empty_table_rejected(oracle):

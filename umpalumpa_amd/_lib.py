@@ -124,6 +124,9 @@ PROTOTYPES = [
     ("umpa_fr_autotune", C.c_int, [_P, _P, _P, _U64, _P, _U64, _P, C.c_int,
                                    C.POINTER(SchedParams)]),
     ("umpa_fr_get_sched_params", C.c_int, [_P, C.POINTER(SchedParams)]),
+    ("umpa_fr_set_sched_params", C.c_int, [_P, C.c_int, C.POINTER(SchedParams)]),
+    ("umpa_fr_brick_schedule", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                         C.POINTER(SchedParams), _P, _U64, C.POINTER(_U64)]),
     ("umpa_fr_batch_autotune", C.c_int, [_P, C.POINTER(FRJob), C.c_uint32, C.c_uint32, C.c_int,
                                          C.c_int, C.POINTER(SchedParams)]),
     ("umpa_fr_batch_set_sched_params", C.c_int, [_P, C.c_int, C.POINTER(SchedParams)]),
@@ -136,6 +139,7 @@ PROTOTYPES = [
     ("umpa_fr_batch_execute", C.c_int, [_P]),
     ("umpa_fr_batch_step", C.c_int, [_P, C.POINTER(FRJob), C.POINTER(_U64), C.POINTER(_U64),
                                      C.c_uint32, C.c_int]),
+    ("umpa_fr_batch_check_jobs", C.c_int, [C.POINTER(FRJob), C.c_uint32, C.c_int]),
     ("umpa_fr_batch_num_steps", C.c_int, [_P, C.POINTER(C.c_uint32)]),
     ("umpa_fr_batch_set_count_updates", C.c_int, [_P, C.c_int]),
     ("umpa_fr_batch_last_updates", C.c_int, [_P, C.POINTER(_U64)]),

@@ -442,6 +442,78 @@ int umpa_fr_get_sched_params(umpa_fr_op *op, umpa_fr_sched_params *out)
   return UMPA_OK;
 }
 
+static bool sched_params_valid(const umpa_fr_sched_params &p)
+{
+  return std::isfinite(p.depthScale) && p.depthScale > 0.f && p.depthScale <= 64.f &&
+         p.maxParts >= 1 && p.maxParts <= 255 && std::isfinite(p.splitAt) && p.splitAt > 0.f &&
+         p.splitAt <= 1024.f;
+}
+
+int umpa_fr_set_sched_params(umpa_fr_op *op, int strategy, const umpa_fr_sched_params *p)
+{
+  if (!op || !p || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH) ||
+      !sched_params_valid(*p))
+    return UMPA_ERR_INVALID_ARG;
+  const int slot = strategy == UMPA_FR_FAST_MATH ? 1 : 0;
+  const umpa::host::SchedParams sp{ p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
+  if (!op->initialized) {
+    op->params[slot] = sp;  // used by the next Init
+    return UMPA_OK;
+  }
+  DeviceGuard g(op->device);
+  if (!g.ok) return UMPA_ERR_DEVICE;
+  try {
+    const auto bricks = umpa::host::make_brick_schedule(
+      (int)op->desc.volume.x, (int)op->desc.volume.y, (int)op->desc.volume.z, slot == 1, sp);
+    int *items = nullptr;
+    hipError_t e = upload_bricks(bricks, &items);
+    if (e != hipSuccess) return from_hip(e);
+    // executes already enqueued on the stream may still read the old schedule
+    e = hipStreamSynchronize(op->stream);
+    if (e != hipSuccess) {
+      (void)hipFree(items);
+      return from_hip(e);
+    }
+    int *&dst = slot == 1 ? op->dBricksSplit : op->dBricks;
+    (void)hipFree(dst);
+    dst = items;
+    (slot == 1 ? op->nBricksSplit : op->nBricks) = (int)bricks.size();
+    op->params[slot] = sp;
+    return UMPA_OK;
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+}
+
+int umpa_fr_brick_schedule(uint32_t vx, uint32_t vy, uint32_t vz, int splitHeavy,
+                           const umpa_fr_sched_params *p, int32_t *items, uint64_t capacity,
+                           uint64_t *count)
+{
+  if (!count || vx == 0 || vy == 0 || vz == 0 || vx > 65535 || vy > 65535 || vz > 65535)
+    return UMPA_ERR_INVALID_ARG;
+  umpa::host::SchedParams sp;
+  if (p) {
+    if (!sched_params_valid(*p)) return UMPA_ERR_INVALID_ARG;
+    sp = { p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
+  }
+  try {
+    const auto bricks =
+      umpa::host::make_brick_schedule((int)vx, (int)vy, (int)vz, splitHeavy != 0, sp);
+    *count = bricks.size();
+    if (!items) return UMPA_OK;
+    if (capacity < bricks.size()) return UMPA_ERR_INVALID_ARG;
+    for (size_t i = 0; i < bricks.size(); ++i) {
+      items[4 * i + 0] = bricks[i].x;
+      items[4 * i + 1] = bricks[i].y;
+      items[4 * i + 2] = bricks[i].z;
+      items[4 * i + 3] = bricks[i].w;
+    }
+    return UMPA_OK;
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+}
+
 int umpa_fr_set_strategy(umpa_fr_op *op, int strategy)
 {
   if (!op || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH))

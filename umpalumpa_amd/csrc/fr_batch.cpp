@@ -261,6 +261,26 @@ hipError_t enqueue(umpa_fr_batch *b, const LaunchImage &li, char *base, bool cou
   return umpa::launch_fr_insert(unused, l, v, b->numCUs, b->stream);
 }
 
+// The volumes and weights the jobs accumulate into must be pairwise disjoint: a wavefront
+// loads, sums and stores its brick without atomics, so two work items of one launch writing
+// the same voxel would lose updates.
+bool outputs_disjoint(const std::vector<JobInfo> &infos)
+{
+  std::vector<std::pair<uintptr_t, uintptr_t>> ranges;  // [begin, end) in bytes
+  ranges.reserve(2 * infos.size());
+  for (const JobInfo &j : infos) {
+    const uint64_t n = (uint64_t)j.rec.vx * (uint64_t)j.rec.vy * (uint64_t)j.rec.vz;
+    const uintptr_t v = reinterpret_cast<uintptr_t>(j.rec.volume);
+    const uintptr_t w = reinterpret_cast<uintptr_t>(j.rec.weights);
+    ranges.emplace_back(v, v + n * 8);
+    ranges.emplace_back(w, w + n * 4);
+  }
+  std::sort(ranges.begin(), ranges.end());
+  for (size_t i = 1; i < ranges.size(); ++i)
+    if (ranges[i].first < ranges[i - 1].second) return false;
+  return true;
+}
+
 int validate_all(const umpa_fr_job *jobs, uint32_t nJobs, int strategy, std::vector<JobInfo> &infos)
 {
   if (!jobs || nJobs == 0 || nJobs > 32767) return UMPA_ERR_INVALID_ARG;
@@ -269,6 +289,7 @@ int validate_all(const umpa_fr_job *jobs, uint32_t nJobs, int strategy, std::vec
     const int st = check_job(jobs[i], strategy, infos[i]);
     if (st != UMPA_OK) return st;
   }
+  if (!outputs_disjoint(infos)) return UMPA_ERR_INVALID_ARG;
   return UMPA_OK;
 }
 
@@ -600,6 +621,16 @@ int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy, const umpa_fr
   b->params[strategy == UMPA_FR_FAST_MATH ? 1 : 0] =
     umpa::host::SchedParams{ p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
   return UMPA_OK;
+}
+
+int umpa_fr_batch_check_jobs(const umpa_fr_job *jobs, uint32_t nJobs, int strategy)
+{
+  try {
+    std::vector<JobInfo> infos;
+    return validate_all(jobs, nJobs, strategy, infos);
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
 }
 
 int umpa_fr_batch_num_steps(umpa_fr_batch *b, uint32_t *steps)

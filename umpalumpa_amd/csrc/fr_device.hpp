@@ -16,9 +16,6 @@
 
 namespace umpa {
 
-// Brick of voxels owned by one wavefront: 4 x 4 x 8, two voxels per lane (z and z + 4).
-constexpr int kBrickX = 4, kBrickY = 4, kBrickZ = 8;
-
 __device__ __forceinline__ float clampf_ref(float v, float lo, float hi)
 {
   // clamp<T,U> (fr_common_kernels.hpp:13-17), NaN passes through unchanged

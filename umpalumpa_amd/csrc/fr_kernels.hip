@@ -1005,6 +1005,21 @@ static const void *pick_nb(int nb, bool lds, bool batched)
   }
 }
 
+// Precise Kaiser (kDynamic, alpha > 15): one instantiation per blob order
+// (fr_common_kernels.hpp:129-161); orders outside 0..4 are rejected at Init.
+template <bool kCount>
+static const void *pick_order(int order, bool batched)
+{
+  switch (order) {
+  case 0: return pick_lds<kModePreciseKaiser, 0, 4, false, false, kCount>(false, batched);
+  case 1: return pick_lds<kModePreciseKaiser, 1, 4, false, false, kCount>(false, batched);
+  case 2: return pick_lds<kModePreciseKaiser, 2, 4, false, false, kCount>(false, batched);
+  case 3: return pick_lds<kModePreciseKaiser, 3, 4, false, false, kCount>(false, batched);
+  case 4: return pick_lds<kModePreciseKaiser, 4, 4, false, false, kCount>(false, batched);
+  default: return nullptr;
+  }
+}
+
 static bool table_in_lds(const FrVariant &v, int nTable)
 {
   return v.mode == kModeLookup && nTable <= kMaxLdsTable;
@@ -1031,14 +1046,8 @@ static const void *pick_kernel(const FrVariant &v, int nTable)
     return v.count ? pick_lds<kModeNearest, 0, 4, false, false, true>(false, b)
                    : pick_lds<kModeNearest, 0, 4, false, false, false>(false, b);
   case kModePreciseKaiser:
-    if (v.count) return pick_lds<kModePreciseKaiser, 0, 4, false, false, true>(false, b);
-    switch (v.order) {
-    case 0: return pick_lds<kModePreciseKaiser, 0, 4, false, false, false>(false, b);
-    case 1: return pick_lds<kModePreciseKaiser, 1, 4, false, false, false>(false, b);
-    case 2: return pick_lds<kModePreciseKaiser, 2, 4, false, false, false>(false, b);
-    case 3: return pick_lds<kModePreciseKaiser, 3, 4, false, false, false>(false, b);
-    default: return pick_lds<kModePreciseKaiser, 4, 4, false, false, false>(false, b);
-    }
+    // the counting variant evaluates the same blob order as the plain one
+    return v.count ? pick_order<true>(v.order, b) : pick_order<false>(v.order, b);
   default: return nullptr;
   }
 }

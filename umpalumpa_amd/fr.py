@@ -27,6 +27,7 @@ __all__ = [
     "compute_traverse_space", "traverse_spaces", "generate_traverse_spaces", "BatchJob",
     "FRBatch", "SchedParams", "symmetry_operators", "finalize", "FourierTransform",
     "FFT_FORWARD", "FFT_INVERSE", "FFT_F32", "FFT_F64", "FourierTransformProcessing",
+    "check_batch_jobs", "brick_schedule",
 ]
 
 
@@ -62,6 +63,15 @@ def _ptr(t, what: str = "tensor", nbytes: int = 0):
 
 def _elems(s) -> int:
     return int(s.x) * int(s.y) * int(s.z)
+
+
+def _in_use_on(stream, tensors) -> None:
+    """Marks device tensors as used by work enqueued on `stream` (torch's record_stream): a
+    tensor made on another stream and dropped by the caller right after the call is then not
+    handed out again by the caching allocator until that work has run."""
+    for t in tensors:
+        if t is not None:
+            t.record_stream(stream)
 
 
 def fr_desc(slice_x: int, slice_y: int, n_slices: int, volume: int, n_spaces: int = 0,
@@ -202,6 +212,9 @@ class FourierReconstruction:
         112-byte records, table: float32 device tensor."""
         n_sp = spaces.numel() // 112 if n_spaces is None else n_spaces
         p = self._pointers(fft, scales, spaces, n_sp, table, volume, weights)
+        import torch
+        if self._stream != torch.cuda.current_stream(self.device):
+            _in_use_on(self._stream, (fft, scales, spaces, table, volume, weights))
         lib = _lib.load()
         if scales is None:
             self.last_status = lib.umpa_fr_execute(
@@ -238,6 +251,11 @@ class FourierReconstruction:
                                            C.byref(out)), "umpa_fr_autotune")
         return out
 
+    def set_sched_params(self, strategy: int, p: SchedParams) -> None:
+        """Forces the schedule knobs of `strategy` (umpa_fr_set_sched_params)."""
+        check(_lib.load().umpa_fr_set_sched_params(self._h, strategy, C.byref(p)),
+              "set_sched_params")
+
     def sched_params(self) -> SchedParams:
         out = SchedParams()
         check(_lib.load().umpa_fr_get_sched_params(self._h, C.byref(out)), "sched_params")
@@ -265,6 +283,9 @@ class BatchJob:
     def n_spaces(self) -> int:
         return self.spaces.numel() // 112
 
+    def tensors(self):
+        return (self.fft, self.scales, self.volume, self.weights, self.spaces, self.table)
+
     def to_c(self) -> FRJob:
         n, hy, hx = self.fft.shape[0], self.fft.shape[1], self.fft.shape[2]
         v = self.volume.shape[-1]
@@ -280,6 +301,29 @@ class BatchJob:
                      ptr(self.weights, "weights", 4 * v ** 3),
                      ptr(self.spaces, "spaces", 112 * self.n_spaces),
                      ptr(self.table, "table", 4 * self.n_table))
+
+
+def brick_schedule(vx: int, vy: int, vz: int, split_heavy: bool,
+                   params: SchedParams | None = None) -> np.ndarray:
+    """The insertion's work items (umpa_fr_brick_schedule): int32 [n, 4] rows {x, y, z, flags}
+    with depth = flags & 15, priority bit 4, parts (flags >> 8) & 255, part (flags >> 16)."""
+    lib = _lib.load()
+    n = C.c_uint64(0)
+    p = C.byref(params) if params is not None else None
+    check(lib.umpa_fr_brick_schedule(vx, vy, vz, int(split_heavy), p, None, 0, C.byref(n)),
+          "umpa_fr_brick_schedule")
+    out = np.zeros((n.value, 4), np.int32)
+    check(lib.umpa_fr_brick_schedule(vx, vy, vz, int(split_heavy), p,
+                                     out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)),
+          "umpa_fr_brick_schedule")
+    return out
+
+
+def check_batch_jobs(jobs, strategy: int = FR_EXACT) -> int:
+    """The checks a batch plan / step applies to `jobs`, on the host only
+    (umpa_fr_batch_check_jobs): returns the status code (0 when the jobs are accepted)."""
+    arr = (FRJob * len(jobs))(*[j.to_c() for j in jobs])
+    return int(_lib.load().umpa_fr_batch_check_jobs(arr, len(jobs), strategy))
 
 
 class FRBatch:
@@ -307,6 +351,8 @@ class FRBatch:
         self._follow()
         arr = (FRJob * len(jobs))(*[j.to_c() for j in jobs])
         self._jobs = (jobs, arr)
+        for j in jobs:
+            _in_use_on(self.stream, j.tensors())
         check(_lib.load().umpa_fr_batch_plan(self._h, arr, len(jobs), chunk, strategy,
                                              int(graph)), "umpa_fr_batch_plan")
 
@@ -332,10 +378,17 @@ class FRBatch:
 
     def execute(self) -> None:
         self._follow()
+        if self._jobs is not None:
+            for j in self._jobs[0]:
+                _in_use_on(self.stream, j.tensors())
         check(_lib.load().umpa_fr_batch_execute(self._h), "umpa_fr_batch_execute")
 
     def step(self, jobs, begin, end, strategy: int = FR_EXACT) -> None:
+        """Enqueues one step; the job tensors are marked as in use on the batch stream, so
+        the caller may drop them (e.g. a freshly made slice chunk) as soon as this returns."""
         self._follow()
+        for j in jobs:
+            _in_use_on(self.stream, j.tensors())
         n = len(jobs)
         arr = (FRJob * n)(*[j.to_c() for j in jobs])
         b = (C.c_uint64 * n)(*[int(x) for x in begin])
