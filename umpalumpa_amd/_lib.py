@@ -179,6 +179,14 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         raise RuntimeError(
             f"umpalumpa_amd: HIP library {path} is missing; build it with "
             "`python -c 'import __graft_entry__ as g; g.build()'` (there is no CPU fallback)")
+    try:
+        # torch ships its own HIP runtime.  Loaded first, it is the one this library binds to
+        # (same soname), so the process holds a single runtime and torch's device pointers are
+        # valid here; loaded after this library, torch would map a second runtime beside the
+        # one under /opt/rocm, and device pointers of one would be invalid in the other.
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, res, args in PROTOTYPES:
         fn = getattr(lib, name)
