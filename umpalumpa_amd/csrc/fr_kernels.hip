@@ -251,16 +251,21 @@ __device__ __forceinline__ float candidate_weight(float d2, float r2, int nTable
                                                   const umpa_fr_constants &c)
 {
   if constexpr (kFastMath && kMode == kModeLookup && kTableLds) {
-    // 2^23 + d2 * cIDeltaSqrt rounds to an integer (float spacing 1 in [2^23, 2^24)), so the
-    // position, rounded to the nearest entry, sits in the low mantissa bits: no conversion
-    // instruction.  Clamped into the zero entry nTable (< 2^23), +inf included.
-    // The table's LDS address minus 4 * 0x4B000000 (the bits of 2^23) is one wave-uniform
-    // register, so a candidate's address is a single shift-and-add of the position's bits.
-    const float pos = fminf(__builtin_fmaf(d2, c.cIDeltaSqrt, 8388608.f), 8388608.f + (float)nTable);
+    // The reference's position (int)(d2 * cIDeltaSqrt + 0.5f): the product rounded to float,
+    // then floor(t + 0.5), which v_cvt_rpi_i32_f32 computes in one instruction from the exact
+    // sum; the reference's rounded sum truncates to the same integer for every t >= 0 outside
+    // (0.5 - 2^-25, 0.5) (on the GPU: 2.16 M values, ties k + 0.5 and their neighbours).
+    // Rounding the fused product to the nearest entry instead (one instruction less) moves
+    // entries at rounding boundaries: 4.2e-5 rel-L2 from FRCPU at box 384 with 64 spaces,
+    // whose central voxels carry most of the volume's norm.  Positions past the table and
+    // +inf (saturated to INT_MAX) are clamped into the zero entry nTable, so candidates
+    // outside the blob weigh 0 without a test.
+    const float t = d2 * c.cIDeltaSqrt;
+    int pos;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(pos) : "v"(t));
+    const uint32_t aux = (uint32_t)pos;
     typedef __attribute__((address_space(3))) const float lds_float;
-    const uint32_t biased = __builtin_amdgcn_readfirstlane(
-      (uint32_t)(uintptr_t)(lds_float *)tbl - (0x4B000000u << 2));
-    return *(lds_float *)(uintptr_t)(biased + (__float_as_uint(pos) << 2));
+    return ((lds_float *)tbl)[aux < (uint32_t)nTable ? aux : (uint32_t)nTable];
   } else {
     const bool in = d2 <= r2;
     if constexpr (kMode == kModeLookup) {
