@@ -128,6 +128,23 @@ int umpa_fr_execute_f16(umpa_fr_op *op, const void *fft, const float *scales,
                         const umpa_traverse_space *spaces, uint64_t nSpaces,
                         const float *table);
 
+/* Init + Execute in one call that allocates no memory: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under stream capture, tight serving
+ * loops).  Arguments as umpa_fr_init + umpa_fr_execute (umpa_fr_execute_f16 when `scales` is
+ * not NULL) of an operation on `device` and `stream` with `strategy`; same checks, status
+ * codes and results as that operation with its default schedule.  The work counter and the
+ * work items live in a static device array of 8 MiB per GPU (held until the process ends),
+ * which takes volumes up to 513^3: larger ones give UMPA_ERR_UNSUPPORTED.  A call whose
+ * volume shape (or strategy class) differs from the previous call's on that GPU first waits
+ * for `stream` and copies the new work items in (a blocking copy, so such a call is not
+ * capturable); other calls only enqueue kernels.  All calls on one GPU share the array: calls
+ * that overlap on one GPU (other streams or host threads) race. */
+int umpa_fr_execute_noalloc(int device, void *stream, const umpa_fr_desc *desc,
+                            const umpa_fr_settings *settings, int strategy, const void *fft,
+                            const float *scales, uint64_t fftCount, void *volume,
+                            float *weights, const umpa_traverse_space *spaces,
+                            uint64_t nSpaces, const float *table);
+
 /* Select the kernel strategy (UMPA_FR_EXACT default).  No counterpart in the reference API:
  * FRCUDA picks kernel variants through KTT (fr_cuda.cpp:100-157). */
 int umpa_fr_set_strategy(umpa_fr_op *op, int strategy);

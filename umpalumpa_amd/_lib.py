@@ -107,6 +107,8 @@ PROTOTYPES = [
     ("umpa_fr_init", C.c_int, [_P, C.POINTER(FRDesc), C.POINTER(FRSettings)]),
     ("umpa_fr_execute", C.c_int, [_P, _P, _U64, _P, _P, _P, _U64, _P]),
     ("umpa_fr_execute_f16", C.c_int, [_P, _P, _P, _U64, _P, _P, _P, _U64, _P]),
+    ("umpa_fr_execute_noalloc", C.c_int, [C.c_int, _P, C.POINTER(FRDesc), C.POINTER(FRSettings),
+                                          C.c_int, _P, _P, _U64, _P, _P, _P, _U64, _P]),
     ("umpa_fr_set_strategy", C.c_int, [_P, C.c_int]),
     ("umpa_fr_synchronize", C.c_int, [_P]),
     ("umpa_fr_cleanup", C.c_int, [_P]),

@@ -95,6 +95,34 @@ struct umpa_fp_op {
   umpa_fp_settings settings{};
 };
 
+// IsValid (afr.hpp:47-54) and Strategy1::Init (fr_cpu.cpp:14-19): the checks of Init.
+static int fr_check_init(const umpa_fr_desc &desc, const umpa_fr_settings &s)
+{
+  // odd volume x only
+  if (!size_valid(desc.volume) || !size_valid(desc.fft)) return UMPA_ERR_INVALID_ARG;
+  if (desc.volume.x % 2 == 0) return UMPA_ERR_INVALID_ARG;
+  // the volume is indexed with one extent for y and z (fr_cpu_kernels.hpp:114-115)
+  if (desc.volume.y != desc.volume.z || desc.volume.n != 1) return UMPA_ERR_INVALID_ARG;
+  if (s.interpolation < 0 || s.interpolation > 1 || s.type < 0 || s.type > 1 || s.order < 0 ||
+      s.order > 4 || !(s.blobRadius > 0.f) || !std::isfinite(s.alpha))
+    return UMPA_ERR_INVALID_ARG;
+  if (desc.volume.x > (uint64_t)INT_MAX || desc.volume.y > (uint64_t)INT_MAX ||
+      desc.fft.x > (uint64_t)INT_MAX || desc.fft.y > (uint64_t)INT_MAX ||
+      desc.fft.n > (uint64_t)INT_MAX || desc.blobTable > (uint64_t)INT_MAX)
+    return UMPA_ERR_UNSUPPORTED;
+  // a slice is read through a buffer resource with a 32-bit byte range (complex float pixels)
+  if (desc.fft.x * desc.fft.y * 8 > (uint64_t)INT_MAX) return UMPA_ERR_UNSUPPORTED;
+  return UMPA_OK;
+}
+
+// Widest blob footprint in pixels: floor(2r) + 1 (plus a margin for rounding at the edge),
+// clamped to the unrolled widths 3..6; wider footprints take the kernel's generic path.
+static int fr_footprint(const umpa_fr_settings &s)
+{
+  const int nb = (int)std::floor(2.0 * (double)s.blobRadius + 1e-3) + 1;
+  return nb < 3 ? 3 : (nb > 6 ? 6 : nb);
+}
+
 extern "C" {
 
 int umpa_version(void) { return 100; }
@@ -138,20 +166,8 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
   if (!g.ok) return UMPA_ERR_DEVICE;
   // Init starts from a clean state (basic_operation.hpp:30-31)
   op->release();
-  // IsValid (afr.hpp:47-54) and Strategy1::Init (fr_cpu.cpp:14-19): odd volume x only
-  if (!size_valid(desc->volume) || !size_valid(desc->fft)) return UMPA_ERR_INVALID_ARG;
-  if (desc->volume.x % 2 == 0) return UMPA_ERR_INVALID_ARG;
-  // the volume is indexed with one extent for y and z (fr_cpu_kernels.hpp:114-115)
-  if (desc->volume.y != desc->volume.z || desc->volume.n != 1) return UMPA_ERR_INVALID_ARG;
-  if (s->interpolation < 0 || s->interpolation > 1 || s->type < 0 || s->type > 1 ||
-      s->order < 0 || s->order > 4 || !(s->blobRadius > 0.f) || !std::isfinite(s->alpha))
-    return UMPA_ERR_INVALID_ARG;
-  if (desc->volume.x > (uint64_t)INT_MAX || desc->volume.y > (uint64_t)INT_MAX ||
-      desc->fft.x > (uint64_t)INT_MAX || desc->fft.y > (uint64_t)INT_MAX ||
-      desc->fft.n > (uint64_t)INT_MAX || desc->blobTable > (uint64_t)INT_MAX)
-    return UMPA_ERR_UNSUPPORTED;
-  // a slice is read through a buffer resource with a 32-bit byte range (complex float pixels)
-  if (desc->fft.x * desc->fft.y * 8 > (uint64_t)INT_MAX) return UMPA_ERR_UNSUPPORTED;
+  const int st = fr_check_init(*desc, *s);
+  if (st != UMPA_OK) return st;
   try {
     op->desc = *desc;
     op->settings = *s;
@@ -170,10 +186,7 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
     }
     op->nBricksSplit = (int)split.size();
     op->nBricks = (int)bricks.size();
-    // widest blob footprint in pixels: floor(2r) + 1 (plus a margin for rounding at the edge);
-    // wider footprints take the kernel's generic path
-    const int nb = (int)std::floor(2.0 * (double)s->blobRadius + 1e-3) + 1;
-    op->nb = nb < 3 ? 3 : (nb > 6 ? 6 : nb);
+    op->nb = fr_footprint(*s);
     op->initialized = true;
     return UMPA_OK;
   } catch (const std::bad_alloc &) {
@@ -185,20 +198,20 @@ int umpa_fr_init(umpa_fr_op *op, const umpa_fr_desc *desc, const umpa_fr_setting
   }
 }
 
-// Validation shared by execute and autotune: the reference's Execute preconditions.
-static int fr_check_execute(umpa_fr_op *op, const void *fft, const float *scales, bool half,
-                            uint64_t fftCount, const void *volume, const float *weights,
+// The reference's Execute preconditions on the payload of an operation initialized with
+// (desc, s).
+static int fr_check_payload(const umpa_fr_desc &desc, const umpa_fr_settings &s,
+                            const void *fft, const float *scales, bool half, uint64_t fftCount,
+                            const void *volume, const float *weights,
                             const umpa_traverse_space *spaces, uint64_t nSpaces,
                             const float *table)
 {
-  if (!op) return UMPA_ERR_INVALID_ARG;
-  if (!op->initialized) return UMPA_ERR_NOT_INITIALIZED;
   // BasicOperation::Execute equivalence (basic_operation.hpp:62-63): N may shrink
-  if (fftCount > op->desc.fft.n) return UMPA_ERR_NOT_EQUIVALENT;
+  if (fftCount > desc.fft.n) return UMPA_ERR_NOT_EQUIVALENT;
   // FRCUDA Strategy1::Execute rejects empty payloads (fr_cuda.cpp:165-167)
   if (!fft || !volume || !weights || fftCount == 0) return UMPA_ERR_REJECTED;
   // FRCPU rejects kLookup without a table (fr_cpu.cpp:27-29)
-  if (op->settings.interpolation == UMPA_FR_LOOKUP && (!table || op->desc.blobTable == 0))
+  if (s.interpolation == UMPA_FR_LOOKUP && (!table || desc.blobTable == 0))
     return UMPA_ERR_REJECTED;
   if (nSpaces > (uint64_t)INT_MAX) return UMPA_ERR_UNSUPPORTED;
   if (nSpaces > 0 && !spaces) return UMPA_ERR_INVALID_ARG;
@@ -209,52 +222,80 @@ static int fr_check_execute(umpa_fr_op *op, const void *fft, const float *scales
   return UMPA_OK;
 }
 
-static umpa::FrJob fr_make_job(const umpa_fr_op *op, const void *fft, const float *scales,
-                               uint64_t fftCount, void *volume, float *weights,
-                               const umpa_traverse_space *spaces, uint64_t nSpaces)
+// Validation shared by execute and autotune: the reference's Execute preconditions.
+static int fr_check_execute(umpa_fr_op *op, const void *fft, const float *scales, bool half,
+                            uint64_t fftCount, const void *volume, const float *weights,
+                            const umpa_traverse_space *spaces, uint64_t nSpaces,
+                            const float *table)
+{
+  if (!op) return UMPA_ERR_INVALID_ARG;
+  if (!op->initialized) return UMPA_ERR_NOT_INITIALIZED;
+  return fr_check_payload(op->desc, op->settings, fft, scales, half, fftCount, volume, weights,
+                          spaces, nSpaces, table);
+}
+
+static umpa::FrJob fr_job_record(const umpa_fr_desc &desc, const umpa_fr_settings &s,
+                               const umpa_fr_constants &c, const void *fft,
+                               const float *scales, uint64_t fftCount, void *volume,
+                               float *weights, const umpa_traverse_space *spaces,
+                               uint64_t nSpaces)
 {
   umpa::FrJob a{};
   a.volume = volume;
   a.weights = weights;
   a.fft = fft;
   a.scales = scales;
-  a.xSize = (int)op->desc.fft.x;
-  a.ySize = (int)op->desc.fft.y;
-  a.sliceElems = (int64_t)op->desc.fft.x * (int64_t)op->desc.fft.y;
+  a.xSize = (int)desc.fft.x;
+  a.ySize = (int)desc.fft.y;
+  a.sliceElems = (int64_t)desc.fft.x * (int64_t)desc.fft.y;
   a.nProj = (int)fftCount;
   a.nSpaces = (int)nSpaces;
   a.spaceBegin = 0;
   a.spaceEnd = (int)nSpaces;
   a.spaces = spaces;
-  a.c = op->constants;
-  a.vx = (int)op->desc.volume.x;
-  a.vy = (int)op->desc.volume.y;
-  a.vz = (int)op->desc.volume.z;
-  const bool nearest = op->settings.type == UMPA_FR_FAST;
-  a.cullHalfWidth = nearest ? 1.6f : op->settings.blobRadius;
-  a.cullHalfSpace = nearest ? 0.f : op->settings.blobRadius;
+  a.c = c;
+  a.vx = (int)desc.volume.x;
+  a.vy = (int)desc.volume.y;
+  a.vz = (int)desc.volume.z;
+  const bool nearest = s.type == UMPA_FR_FAST;
+  a.cullHalfWidth = nearest ? 1.6f : s.blobRadius;
+  a.cullHalfSpace = nearest ? 0.f : s.blobRadius;
   return a;
+}
+
+static umpa::FrJob fr_make_job(const umpa_fr_op *op, const void *fft, const float *scales,
+                               uint64_t fftCount, void *volume, float *weights,
+                               const umpa_traverse_space *spaces, uint64_t nSpaces)
+{
+  return fr_job_record(op->desc, op->settings, op->constants, fft, scales, fftCount, volume,
+                       weights, spaces, nSpaces);
+}
+
+static umpa::FrVariant fr_variant_for(const umpa_fr_settings &s, int nb, int strategy,
+                                      bool half, bool count)
+{
+  umpa::FrVariant v{};
+  if (s.type == UMPA_FR_FAST) {
+    v.mode = umpa::kModeNearest;
+  } else if (s.interpolation == UMPA_FR_LOOKUP) {
+    v.mode = umpa::kModeLookup;
+  } else {
+    // FRCPU picks the rational I0 approximation for alpha <= 15 (fr_cpu.cpp:31)
+    v.mode = s.alpha <= 15.f ? umpa::kModeFastKaiser : umpa::kModePreciseKaiser;
+  }
+  v.order = s.order;
+  v.nb = nb;
+  v.half = half;
+  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
+  v.fastMath = half || strategy == UMPA_FR_FAST_MATH;
+  v.count = count;
+  v.batched = false;
+  return v;
 }
 
 static umpa::FrVariant fr_make_variant(const umpa_fr_op *op, bool half)
 {
-  umpa::FrVariant v{};
-  if (op->settings.type == UMPA_FR_FAST) {
-    v.mode = umpa::kModeNearest;
-  } else if (op->settings.interpolation == UMPA_FR_LOOKUP) {
-    v.mode = umpa::kModeLookup;
-  } else {
-    // FRCPU picks the rational I0 approximation for alpha <= 15 (fr_cpu.cpp:31)
-    v.mode = op->settings.alpha <= 15.f ? umpa::kModeFastKaiser : umpa::kModePreciseKaiser;
-  }
-  v.order = op->settings.order;
-  v.nb = op->nb;
-  v.half = half;
-  // bit-exact runs keep every brick on one wavefront; fast-math runs may split central bricks
-  v.fastMath = half || op->strategy == UMPA_FR_FAST_MATH;
-  v.count = op->countUpdates;
-  v.batched = false;
-  return v;
+  return fr_variant_for(op->settings, op->nb, op->strategy, half, op->countUpdates);
 }
 
 static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales, bool half,
@@ -298,6 +339,55 @@ int umpa_fr_execute(umpa_fr_op *op, const void *fft, uint64_t fftCount, void *vo
 {
   return fr_execute_impl(op, fft, nullptr, false, fftCount, volume, weights, spaces, nSpaces,
                          table);
+}
+
+int umpa_fr_execute_noalloc(int device, void *stream, const umpa_fr_desc *desc,
+                            const umpa_fr_settings *s, int strategy, const void *fft,
+                            const float *scales, uint64_t fftCount, void *volume,
+                            float *weights, const umpa_traverse_space *spaces,
+                            uint64_t nSpaces, const float *table)
+{
+  if (!desc || !s || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH))
+    return UMPA_ERR_INVALID_ARG;
+  int st = fr_check_init(*desc, *s);
+  if (st != UMPA_OK) return st;
+  const bool half = scales != nullptr;
+  st = fr_check_payload(*desc, *s, fft, scales, half, fftCount, volume, weights, spaces,
+                        nSpaces, table);
+  if (st != UMPA_OK) return st;
+  const umpa::FrVariant v = fr_variant_for(*s, fr_footprint(*s), strategy, half, false);
+  if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
+  try {
+    // the schedule a handle would build at Init (default knobs): it must fit the static
+    // scratch, which holds volumes up to 513^3
+    const auto bricks = umpa::host::make_brick_schedule(
+      (int)desc->volume.x, (int)desc->volume.y, (int)desc->volume.z, v.fastMath,
+      umpa::host::SchedParams());
+    if (bricks.size() > umpa::kNoAllocMaxItems) return UMPA_ERR_UNSUPPORTED;
+    if (nSpaces == 0) return UMPA_OK;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count ||
+        device >= umpa::kNoAllocMaxDevices)
+      return UMPA_ERR_DEVICE;
+    DeviceGuard g(device);
+    if (!g.ok) return UMPA_ERR_DEVICE;
+    std::vector<int> items(bricks.size() * 4);
+    for (size_t i = 0; i < bricks.size(); ++i) {
+      items[4 * i + 0] = bricks[i].x;
+      items[4 * i + 1] = bricks[i].y;
+      items[4 * i + 2] = bricks[i].z;
+      items[4 * i + 3] = bricks[i].w;
+    }
+    const umpa_fr_constants c =
+      umpa::host::create_constants(desc->volume.x, desc->volume.y, desc->blobTable, *s);
+    const umpa::FrJob a =
+      fr_job_record(*desc, *s, c, fft, scales, fftCount, volume, weights, spaces, nSpaces);
+    return from_hip(umpa::launch_fr_insert_noalloc(
+      device, a, v, items.data(), (int)bricks.size(), v.fastMath, table,
+      (int)desc->blobTable, query_cus(device), static_cast<hipStream_t>(stream)));
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
 }
 
 int umpa_fr_execute_f16(umpa_fr_op *op, const void *fft, const float *scales,

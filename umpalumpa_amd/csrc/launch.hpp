@@ -65,6 +65,17 @@ hipError_t prepare_fr_insert(const FrVariant &v, int nTable);
 hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v, int numCUs,
                             hipStream_t st);
 
+// Allocation-free insertion: the work counter and the work items live in a static device
+// array of kNoAllocScratchBytes per GPU (a 256-byte header, then int4 items).  `items` is the
+// host schedule of the job's volume (make_brick_schedule with default knobs, `split` for the
+// fast-math schedules); it is uploaded only when the array holds another one.
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+constexpr size_t kNoAllocMaxItems = (kNoAllocScratchBytes - 256) / 16;
+constexpr int kNoAllocMaxDevices = 64;
+hipError_t launch_fr_insert_noalloc(int device, const FrJob &job, const FrVariant &v,
+                                    const int *items, int nItems, bool split, const float *table,
+                                    int nTable, int numCUs, hipStream_t st);
+
 // Finalization of a (side^3) volume: optional Hermitian completion and the FP64 division by
 // the weights, into `out` (complex float).
 hipError_t launch_fr_finalize(const void *volume, const float *weights, int side, void *out,
