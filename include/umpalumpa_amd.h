@@ -235,13 +235,35 @@ void umpa_fr_batch_destroy(umpa_fr_batch *b);
  * caller's slices, spaces and table, keeps the fastest for the handle, and records it in
  * $UMPA_TUNING_DIR/fr_tuning.txt (default $HOME/.cache/umpalumpa_amd) under a key of the
  * device, strategy, precision, footprint and problem shape; with useCache a recorded result is
- * reused without timing.  Results are the same under every schedule for UMPA_FR_EXACT. */
+ * reused without timing.  Results are the same, bit for bit, under every schedule and kernel
+ * knob for UMPA_FR_EXACT. */
 typedef struct umpa_fr_sched_params {
   float depthScale;  /* bricks 8x8x{8,4,2,1} beyond radius {48,24,12} x depthScale */
   int32_t priority;  /* heavy bricks at raised wave priority */
   int32_t maxParts;  /* split heavy bricks into at most this many slice ranges (fast math) */
   float splitAt;     /* ... above splitAt x one wavefront's fair share of the launch */
+  /* Kernel knobs; 0 selects the default in each.  Batches use their own kernel with the table
+   * in LDS and honour boxFixed only. */
+  int32_t kernel;      /* UMPA_FR_KERNEL_*: insertion kernel; one that does not cover the
+                          variant (mode, strategy, footprint) runs as UMPA_FR_KERNEL_PAIR */
+  int32_t tableGlobal; /* 1: blob table read from global memory instead of LDS */
+  int32_t boxFixed;    /* 1: fast-math candidate boxes stay at the blob's bounding box instead
+                          of sliding inside the slice (no masking of out-of-box candidates) */
 } umpa_fr_sched_params;
+
+enum {
+  UMPA_FR_KERNEL_AUTO = 0,   /* staged for UMPA_FR_EXACT, carried for fast-math FP32, else pair */
+  UMPA_FR_KERNEL_PAIR = 1,   /* one queue of slab voxels per (brick, space), 16 or 12 waves */
+  UMPA_FR_KERNEL_CARRY = 2,  /* queues carried across spaces (fast math, footprint 3-4) */
+  UMPA_FR_KERNEL_STAGED = 3  /* cross-space ring of entries (exact FP32, footprint 3-4) */
+};
+
+/* The candidates umpa_fr_autotune times under `strategy` (UMPA_FR_EXACT: every schedule with
+ * every kernel knob; UMPA_FR_FAST_MATH: the schedules with default kernel knobs, then the
+ * kernel knobs over the default schedule, which the tuner applies to the fastest schedule).
+ * out may be NULL to query *count, else capacity must hold them. */
+int umpa_fr_tuning_candidates(int strategy, umpa_fr_sched_params *out, uint64_t capacity,
+                              uint64_t *count);
 
 /* Tunes the schedule the op uses under its current strategy for inputs like these (as for
  * umpa_fr_execute; scales != NULL for FP16 slices).  Not under graph capture (allocates). */

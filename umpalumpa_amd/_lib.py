@@ -74,12 +74,19 @@ class FRJob(C.Structure):
 
 
 class SchedParams(C.Structure):
-    """umpa_fr_sched_params: the autotuner's schedule knobs."""
+    """umpa_fr_sched_params: the autotuner's schedule and kernel knobs (0 = default for the
+    kernel knobs)."""
     _fields_ = [("depthScale", C.c_float), ("priority", C.c_int32), ("maxParts", C.c_int32),
-                ("splitAt", C.c_float)]
+                ("splitAt", C.c_float), ("kernel", C.c_int32), ("tableGlobal", C.c_int32),
+                ("boxFixed", C.c_int32)]
 
     def as_tuple(self):
-        return (self.depthScale, self.priority, self.maxParts, self.splitAt)
+        return (self.depthScale, self.priority, self.maxParts, self.splitAt, self.kernel,
+                self.tableGlobal, self.boxFixed)
+
+
+# umpa_fr_sched_params.kernel
+FR_KERNEL_AUTO, FR_KERNEL_PAIR, FR_KERNEL_CARRY, FR_KERNEL_STAGED = 0, 1, 2, 3
 
 
 class FFTDesc(C.Structure):
@@ -127,6 +134,8 @@ PROTOTYPES = [
                                    C.POINTER(SchedParams)]),
     ("umpa_fr_get_sched_params", C.c_int, [_P, C.POINTER(SchedParams)]),
     ("umpa_fr_set_sched_params", C.c_int, [_P, C.c_int, C.POINTER(SchedParams)]),
+    ("umpa_fr_tuning_candidates", C.c_int, [C.c_int, C.POINTER(SchedParams), _U64,
+                                            C.POINTER(_U64)]),
     ("umpa_fr_brick_schedule", C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                          C.POINTER(SchedParams), _P, _U64, C.POINTER(_U64)]),
     ("umpa_fr_batch_autotune", C.c_int, [_P, C.POINTER(FRJob), C.c_uint32, C.c_uint32, C.c_int,
@@ -194,7 +203,7 @@ def load(path: str = LIB_PATH) -> C.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.umpa_version() < 100:
+    if lib.umpa_version() < 101:   # 101: umpa_fr_sched_params with kernel knobs
         raise RuntimeError("umpalumpa_amd: library version mismatch")
     _lib = lib
     return lib

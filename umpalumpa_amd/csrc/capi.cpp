@@ -117,15 +117,11 @@ static int fr_check_init(const umpa_fr_desc &desc, const umpa_fr_settings &s)
 
 // Widest blob footprint in pixels: floor(2r) + 1 (plus a margin for rounding at the edge),
 // clamped to the unrolled widths 3..6; wider footprints take the kernel's generic path.
-static int fr_footprint(const umpa_fr_settings &s)
-{
-  const int nb = (int)std::floor(2.0 * (double)s.blobRadius + 1e-3) + 1;
-  return nb < 3 ? 3 : (nb > 6 ? 6 : nb);
-}
+static int fr_footprint(const umpa_fr_settings &s) { return umpa::host::footprint(s); }
 
 extern "C" {
 
-int umpa_version(void) { return 100; }
+int umpa_version(void) { return 101; }
 
 const char *umpa_status_string(int status)
 {
@@ -238,7 +234,7 @@ static umpa::FrJob fr_job_record(const umpa_fr_desc &desc, const umpa_fr_setting
                                const umpa_fr_constants &c, const void *fft,
                                const float *scales, uint64_t fftCount, void *volume,
                                float *weights, const umpa_traverse_space *spaces,
-                               uint64_t nSpaces)
+                               uint64_t nSpaces, bool boxFixed)
 {
   umpa::FrJob a{};
   a.volume = volume;
@@ -260,7 +256,14 @@ static umpa::FrJob fr_job_record(const umpa_fr_desc &desc, const umpa_fr_setting
   const bool nearest = s.type == UMPA_FR_FAST;
   a.cullHalfWidth = nearest ? 1.6f : s.blobRadius;
   a.cullHalfSpace = nearest ? 0.f : s.blobRadius;
+  a.shiftBox = !boxFixed && umpa::host::box_can_shift(s, desc.fft.x, desc.fft.y) ? 1 : 0;
   return a;
+}
+
+// The knobs' slot of an execution: 1 for the fast-math schedules (FP32 fast math, FP16)
+static int fr_slot(const umpa_fr_op *op, bool half)
+{
+  return (half || op->strategy == UMPA_FR_FAST_MATH) ? 1 : 0;
 }
 
 static umpa::FrJob fr_make_job(const umpa_fr_op *op, const void *fft, const float *scales,
@@ -268,11 +271,12 @@ static umpa::FrJob fr_make_job(const umpa_fr_op *op, const void *fft, const floa
                                const umpa_traverse_space *spaces, uint64_t nSpaces)
 {
   return fr_job_record(op->desc, op->settings, op->constants, fft, scales, fftCount, volume,
-                       weights, spaces, nSpaces);
+                       weights, spaces, nSpaces, op->params[fr_slot(op, scales != nullptr)].boxFixed);
 }
 
 static umpa::FrVariant fr_variant_for(const umpa_fr_settings &s, int nb, int strategy,
-                                      bool half, bool count)
+                                      bool half, bool count,
+                                      const umpa::host::SchedParams &knobs)
 {
   umpa::FrVariant v{};
   if (s.type == UMPA_FR_FAST) {
@@ -290,12 +294,15 @@ static umpa::FrVariant fr_variant_for(const umpa_fr_settings &s, int nb, int str
   v.fastMath = half || strategy == UMPA_FR_FAST_MATH;
   v.count = count;
   v.batched = false;
+  v.kernel = knobs.kernel;
+  v.tableGlobal = knobs.tableGlobal;
   return v;
 }
 
 static umpa::FrVariant fr_make_variant(const umpa_fr_op *op, bool half)
 {
-  return fr_variant_for(op->settings, op->nb, op->strategy, half, op->countUpdates);
+  return fr_variant_for(op->settings, op->nb, op->strategy, half, op->countUpdates,
+                        op->params[fr_slot(op, half)]);
 }
 
 static int fr_execute_impl(umpa_fr_op *op, const void *fft, const float *scales, bool half,
@@ -355,7 +362,8 @@ int umpa_fr_execute_noalloc(int device, void *stream, const umpa_fr_desc *desc,
   st = fr_check_payload(*desc, *s, fft, scales, half, fftCount, volume, weights, spaces,
                         nSpaces, table);
   if (st != UMPA_OK) return st;
-  const umpa::FrVariant v = fr_variant_for(*s, fr_footprint(*s), strategy, half, false);
+  const umpa::FrVariant v =
+    fr_variant_for(*s, fr_footprint(*s), strategy, half, false, umpa::host::SchedParams());
   if (half && v.mode != umpa::kModeLookup) return UMPA_ERR_UNSUPPORTED;
   try {
     // the schedule a handle would build at Init (default knobs): it must fit the static
@@ -381,7 +389,7 @@ int umpa_fr_execute_noalloc(int device, void *stream, const umpa_fr_desc *desc,
     const umpa_fr_constants c =
       umpa::host::create_constants(desc->volume.x, desc->volume.y, desc->blobTable, *s);
     const umpa::FrJob a =
-      fr_job_record(*desc, *s, c, fft, scales, fftCount, volume, weights, spaces, nSpaces);
+      fr_job_record(*desc, *s, c, fft, scales, fftCount, volume, weights, spaces, nSpaces, false);
     return from_hip(umpa::launch_fr_insert_noalloc(
       device, a, v, items.data(), (int)bricks.size(), v.fastMath, table,
       (int)desc->blobTable, query_cus(device), static_cast<hipStream_t>(stream)));
@@ -420,10 +428,11 @@ static int fr_autotune_impl(umpa_fr_op *op, const void *fft, const float *scales
   // spaces bucketed to a power of two
   uint64_t bucket = 1;
   while (bucket < nSpaces) bucket <<= 1;
-  const std::string key = "op:cu" + std::to_string(op->numCUs) + ":m" + std::to_string(v.mode) +
-                          ":f" + std::to_string(slot) + ":h" + std::to_string(half ? 1 : 0) +
-                          ":nb" + std::to_string(v.nb) + ":" + std::to_string(vx) + "x" +
-                          std::to_string(bucket);
+  // "k2": the format of lines with kernel knobs
+  const std::string key = "k2:op:cu" + std::to_string(op->numCUs) + ":m" +
+                          std::to_string(v.mode) + ":f" + std::to_string(slot) + ":h" +
+                          std::to_string(half ? 1 : 0) + ":nb" + std::to_string(v.nb) + ":" +
+                          std::to_string(vx) + "x" + std::to_string(bucket);
   umpa::host::SchedParams best = op->params[slot];
   bool found = useCache && umpa::host::tuning_cache_lookup(key, best);
   hipError_t e = hipSuccess;
@@ -440,38 +449,48 @@ static int fr_autotune_impl(umpa_fr_op *op, const void *fft, const float *scales
     if (e == hipSuccess) e = hipMemsetAsync(wts, 0, n * 4, op->stream);
     if (e == hipSuccess) e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
-    if (e == hipSuccess) e = umpa::prepare_fr_insert(v, (int)op->desc.blobTable);
-    const umpa::FrJob a = fr_make_job(op, fft, scales, fftCount, vol, static_cast<float *>(wts),
-                                      spaces, nSpaces);
-    try {
-      for (const auto &cand : umpa::host::tuning_candidates(v.fastMath)) {
-        if (e != hipSuccess) break;
-        const auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, v.fastMath, cand);
-        if (items) (void)hipFree(items);
-        items = nullptr;
-        e = upload_bricks(bricks, &items);
-        umpa::FrLaunch l{};
-        l.items = items;
-        l.nItems = (int)bricks.size();
-        l.counter = op->dCounter;
-        l.updates = op->dUpdates;
-        l.table = table;
-        l.nTable = (int)op->desc.blobTable;
-        double ms = 1e300;
-        for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
-          e = hipEventRecord(t0, op->stream);
-          if (e == hipSuccess) e = umpa::launch_fr_insert(a, l, v, op->numCUs, op->stream);
-          if (e == hipSuccess) e = hipEventRecord(t1, op->stream);
-          if (e == hipSuccess) e = hipEventSynchronize(t1);
-          float t = 0.f;
-          if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
-          if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms up
-        }
-        if (e == hipSuccess && ms < bestMs) {
-          bestMs = ms;
-          best = cand;
-        }
+    // one candidate: its schedule, kernel and box, the best of two runs after a warm-up
+    auto time_candidate = [&](const umpa::host::SchedParams &cand) {
+      umpa::FrVariant vc = v;
+      vc.kernel = cand.kernel;
+      vc.tableGlobal = cand.tableGlobal;
+      const umpa::FrJob a =
+        fr_job_record(op->desc, op->settings, op->constants, fft, scales, fftCount, vol,
+                      static_cast<float *>(wts), spaces, nSpaces, cand.boxFixed);
+      const auto bricks = umpa::host::make_brick_schedule(vx, vy, vz, v.fastMath, cand);
+      if (items) (void)hipFree(items);
+      items = nullptr;
+      e = upload_bricks(bricks, &items);
+      if (e == hipSuccess) e = umpa::prepare_fr_insert(vc, (int)op->desc.blobTable);
+      umpa::FrLaunch l{};
+      l.items = items;
+      l.nItems = (int)bricks.size();
+      l.counter = op->dCounter;
+      l.updates = op->dUpdates;
+      l.table = table;
+      l.nTable = (int)op->desc.blobTable;
+      double ms = 1e300;
+      for (int rep = 0; rep < 3 && e == hipSuccess; ++rep) {
+        e = hipEventRecord(t0, op->stream);
+        if (e == hipSuccess) e = umpa::launch_fr_insert(a, l, vc, op->numCUs, op->stream);
+        if (e == hipSuccess) e = hipEventRecord(t1, op->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, t0, t1);
+        if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms up
       }
+      if (e == hipSuccess && ms < bestMs) {
+        bestMs = ms;
+        best = cand;
+      }
+    };
+    try {
+      // schedules (exact: with every kernel knob), then the kernel knobs over the best one
+      for (const auto &cand : umpa::host::tuning_candidates(v.fastMath))
+        if (e == hipSuccess) time_candidate(cand);
+      const umpa::host::SchedParams sched = best;
+      for (const auto &cand : umpa::host::kernel_candidates(sched, v.fastMath))
+        if (e == hipSuccess) time_candidate(cand);
     } catch (const std::bad_alloc &) {
       e = hipErrorOutOfMemory;
     }
@@ -504,7 +523,7 @@ static int fr_autotune_impl(umpa_fr_op *op, const void *fft, const float *scales
     return UMPA_ERR_OUT_OF_MEMORY;
   }
   op->params[slot] = best;
-  if (chosen) *chosen = { best.depthScale, best.priority ? 1 : 0, best.maxParts, best.splitAt };
+  if (chosen) *chosen = umpa::host::to_c(best);
   return UMPA_OK;
 }
 
@@ -527,25 +546,17 @@ int umpa_fr_get_sched_params(umpa_fr_op *op, umpa_fr_sched_params *out)
 {
   if (!op || !out) return UMPA_ERR_INVALID_ARG;
   const int slot = op->strategy == UMPA_FR_FAST_MATH ? 1 : 0;
-  const auto &p = op->params[slot];
-  *out = { p.depthScale, p.priority ? 1 : 0, p.maxParts, p.splitAt };
+  *out = umpa::host::to_c(op->params[slot]);
   return UMPA_OK;
-}
-
-static bool sched_params_valid(const umpa_fr_sched_params &p)
-{
-  return std::isfinite(p.depthScale) && p.depthScale > 0.f && p.depthScale <= 64.f &&
-         p.maxParts >= 1 && p.maxParts <= 255 && std::isfinite(p.splitAt) && p.splitAt > 0.f &&
-         p.splitAt <= 1024.f;
 }
 
 int umpa_fr_set_sched_params(umpa_fr_op *op, int strategy, const umpa_fr_sched_params *p)
 {
   if (!op || !p || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH) ||
-      !sched_params_valid(*p))
+      !umpa::host::sched_params_valid(*p))
     return UMPA_ERR_INVALID_ARG;
   const int slot = strategy == UMPA_FR_FAST_MATH ? 1 : 0;
-  const umpa::host::SchedParams sp{ p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
+  const umpa::host::SchedParams sp = umpa::host::from_c(*p);
   if (!op->initialized) {
     op->params[slot] = sp;  // used by the next Init
     return UMPA_OK;
@@ -575,6 +586,26 @@ int umpa_fr_set_sched_params(umpa_fr_op *op, int strategy, const umpa_fr_sched_p
   }
 }
 
+int umpa_fr_tuning_candidates(int strategy, umpa_fr_sched_params *out, uint64_t capacity,
+                              uint64_t *count)
+{
+  if (!count || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH))
+    return UMPA_ERR_INVALID_ARG;
+  try {
+    const bool fast = strategy == UMPA_FR_FAST_MATH;
+    auto all = umpa::host::tuning_candidates(fast);
+    const auto second = umpa::host::kernel_candidates(umpa::host::SchedParams(), fast);
+    all.insert(all.end(), second.begin(), second.end());
+    *count = all.size();
+    if (!out) return UMPA_OK;
+    if (capacity < all.size()) return UMPA_ERR_INVALID_ARG;
+    for (size_t i = 0; i < all.size(); ++i) out[i] = umpa::host::to_c(all[i]);
+    return UMPA_OK;
+  } catch (const std::bad_alloc &) {
+    return UMPA_ERR_OUT_OF_MEMORY;
+  }
+}
+
 int umpa_fr_brick_schedule(uint32_t vx, uint32_t vy, uint32_t vz, int splitHeavy,
                            const umpa_fr_sched_params *p, int32_t *items, uint64_t capacity,
                            uint64_t *count)
@@ -583,8 +614,8 @@ int umpa_fr_brick_schedule(uint32_t vx, uint32_t vy, uint32_t vz, int splitHeavy
     return UMPA_ERR_INVALID_ARG;
   umpa::host::SchedParams sp;
   if (p) {
-    if (!sched_params_valid(*p)) return UMPA_ERR_INVALID_ARG;
-    sp = { p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
+    if (!umpa::host::sched_params_valid(*p)) return UMPA_ERR_INVALID_ARG;
+    sp = umpa::host::from_c(*p);
   }
   try {
     const auto bricks =

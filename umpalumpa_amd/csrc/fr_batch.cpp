@@ -85,8 +85,7 @@ int check_job(const umpa_fr_job &j, int strategy, JobInfo &out)
   if (v.mode == umpa::kModeLookup && d.blobTable > (uint64_t)umpa::kFrMaxLdsTable)
     return UMPA_ERR_UNSUPPORTED;
   v.order = s.order;
-  const int nb = (int)std::floor(2.0 * (double)s.blobRadius + 1e-3) + 1;
-  v.nb = nb < 3 ? 3 : (nb > 6 ? 6 : nb);
+  v.nb = umpa::host::footprint(s);
   v.half = half;
   v.fastMath = half || strategy == UMPA_FR_FAST_MATH;
   v.count = false;
@@ -109,6 +108,7 @@ int check_job(const umpa_fr_job &j, int strategy, JobInfo &out)
   a.c = umpa::host::create_constants(d.volume.x, d.volume.y, d.blobTable, s);
   a.cullHalfWidth = nearest ? 1.6f : s.blobRadius;
   a.cullHalfSpace = nearest ? 0.f : s.blobRadius;
+  a.shiftBox = umpa::host::box_can_shift(s, d.fft.x, d.fft.y) ? 1 : 0;
   out.v = v;
   out.rec = a;
   out.table = v.mode == umpa::kModeLookup ? j.table : nullptr;
@@ -144,7 +144,8 @@ struct umpa_fr_batch {
   bool count = false;
   unsigned long long lastUpdates = 0;
   unsigned long long *dUpdates = nullptr;
-  std::map<std::tuple<int, int, int, float>, std::vector<umpa::host::BrickCand>> cands;
+  std::map<std::tuple<int, int, int, float>, umpa::host::BrickSet> cands;
+  std::vector<umpa::host::BrickOrigin> schedBuf;  // make_schedule's output, reused
   umpa::host::SchedParams params[2];  // schedule knobs for exact / fast-math launches
 
   // static plan
@@ -167,12 +168,12 @@ struct umpa_fr_batch {
   std::deque<std::pair<uint64_t, hipEvent_t>> tickets;
   uint64_t nextTicket = 1;
 
-  const std::vector<umpa::host::BrickCand> &candidates(int vx, int vy, int vz, float depth)
+  const umpa::host::BrickSet &candidates(int vx, int vy, int vz, float depth)
   {
     auto key = std::make_tuple(vx, vy, vz, depth);
     auto it = cands.find(key);
     if (it == cands.end())
-      it = cands.emplace(key, umpa::host::brick_candidates(vx, vy, vz, depth)).first;
+      it = cands.emplace(key, umpa::host::make_brick_set(vx, vy, vz, depth)).first;
     return it->second;
   }
 
@@ -210,11 +211,13 @@ void build_step(umpa_fr_batch *b, const std::vector<JobInfo> &infos,
       umpa::FrJob rec = infos[ji].rec;
       rec.spaceBegin = s0;
       rec.spaceEnd = s1;
+      if (sp.boxFixed) rec.shiftBox = 0;
       recs.push_back(rec);
       work.push_back({ &b->candidates(rec.vx, rec.vy, rec.vz, sp.depthScale), (int)k,
                        (double)(s1 - s0) });
     }
-    const auto items = umpa::host::make_schedule(work, first.v.fastMath, sp);
+    umpa::host::make_schedule(work, first.v.fastMath, sp, b->schedBuf);
+    const auto &items = b->schedBuf;
     LaunchImage li{};
     li.v = first.v;
     li.table = first.table;
@@ -521,7 +524,7 @@ static int batch_autotune_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32
     shape.emplace_back(infos[i].rec.vx, s1);
   }
   std::sort(shape.begin(), shape.end());
-  std::string key = "batch:cu" + std::to_string(b->numCUs) + ":m" +
+  std::string key = "k2:batch:cu" + std::to_string(b->numCUs) + ":m" +
                     std::to_string(infos[0].v.mode) + ":f" + std::to_string(fastMath ? 1 : 0) +
                     ":h" + std::to_string(infos[0].v.half ? 1 : 0) + ":nb" +
                     std::to_string(infos[0].v.nb) + ":";
@@ -529,7 +532,7 @@ static int batch_autotune_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32
   umpa::host::SchedParams best;
   auto report = [&](const umpa::host::SchedParams &p) {
     b->params[fastMath ? 1 : 0] = p;
-    if (chosen) *chosen = { p.depthScale, p.priority ? 1 : 0, p.maxParts, p.splitAt };
+    if (chosen) *chosen = umpa::host::to_c(p);
     return UMPA_OK;
   };
   if (useCache && umpa::host::tuning_cache_lookup(key, best)) return report(best);
@@ -557,9 +560,28 @@ static int batch_autotune_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32
   if (e == hipSuccess) e = hipEventCreate(&t0);
   if (e == hipSuccess) e = hipEventCreate(&t1);
   double bestMs = 1e300;
+  // batches run their own kernel with the table in LDS: of the kernel knobs only boxFixed
+  // applies, tried (fast math) over the fastest schedule
+  std::vector<umpa::host::SchedParams> cands;
+  for (auto c : umpa::host::tuning_candidates(fastMath)) {
+    c.kernel = UMPA_FR_KERNEL_AUTO;
+    c.tableGlobal = false;
+    const bool dup = std::any_of(cands.begin(), cands.end(), [&](const auto &o) {
+      return o.depthScale == c.depthScale && o.priority == c.priority &&
+             o.maxParts == c.maxParts && o.splitAt == c.splitAt;
+    });
+    if (!dup) cands.push_back(c);
+  }
+  const size_t nSched = cands.size(), nCand = nSched + (fastMath ? 1 : 0);
   try {
-    for (const auto &cand : umpa::host::tuning_candidates(fastMath)) {
+    for (size_t ci = 0; ci < nCand; ++ci) {
       if (e != hipSuccess) break;
+      if (ci == nSched) {  // the fastest schedule with fixed boxes
+        umpa::host::SchedParams c = best;
+        c.boxFixed = true;
+        cands.push_back(c);
+      }
+      const umpa::host::SchedParams cand = cands[ci];
       std::vector<char> blob;
       std::vector<LaunchImage> launches;
       build_step(b, infos, ranges, blob, launches, &cand);
@@ -616,10 +638,9 @@ int umpa_fr_batch_autotune(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32_t n
 int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy, const umpa_fr_sched_params *p)
 {
   if (!b || !p || (strategy != UMPA_FR_EXACT && strategy != UMPA_FR_FAST_MATH) ||
-      !(p->depthScale > 0.f) || p->maxParts < 1 || p->maxParts > 255 || !(p->splitAt > 0.f))
+      !umpa::host::sched_params_valid(*p))
     return UMPA_ERR_INVALID_ARG;
-  b->params[strategy == UMPA_FR_FAST_MATH ? 1 : 0] =
-    umpa::host::SchedParams{ p->depthScale, p->priority != 0, p->maxParts, p->splitAt };
+  b->params[strategy == UMPA_FR_FAST_MATH ? 1 : 0] = umpa::host::from_c(*p);
   return UMPA_OK;
 }
 

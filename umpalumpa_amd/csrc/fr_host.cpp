@@ -9,6 +9,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <functional>
 #include <fstream>
 #include <limits>
 #include <map>
@@ -223,6 +225,25 @@ void compute_traverse_space(uint32_t imgSizeX, uint32_t imgSizeY, const float tr
   space.weight = weight;
 }
 
+int footprint(const umpa_fr_settings &s)
+{
+  const int nb = (int)std::floor(2.0 * (double)s.blobRadius + 1e-3) + 1;
+  return nb < 3 ? 3 : (nb > 6 ? 6 : nb);
+}
+
+bool box_can_shift(const umpa_fr_settings &s, uint64_t sliceX, uint64_t sliceY)
+{
+  const int need = (int)std::floor(2.0 * (double)s.blobRadius + 1e-3) + 1;
+  // the box width the kernel runs with: the footprint for table lookups, 4 for the other
+  // blob modes (which fall back to the general gather past it)
+  const bool lookup = s.type != UMPA_FR_FAST && s.interpolation == UMPA_FR_LOOKUP;
+  const int nb = lookup ? footprint(s) : 4;
+  // positions below 8192 round qx - r and qx + r by at most 2^-11 together, inside the
+  // 1e-3 margin of `need`, so the bounding box never has more than `need` columns or rows
+  return need <= nb && sliceX >= (uint64_t)nb && sliceY >= (uint64_t)nb && sliceX <= 8192 &&
+         sliceY <= 8192;
+}
+
 std::vector<BrickCand> brick_candidates(int vx, int vy, int vz, float depthScale)
 {
   // Bricks are 8 x 8 x D columns, D in {8, 4, 2, 1}.  A voxel at distance R from the centre
@@ -256,50 +277,165 @@ std::vector<BrickCand> brick_candidates(int vx, int vy, int vz, float depthScale
   return cand;
 }
 
-std::vector<BrickOrigin> make_schedule(const std::vector<JobWork> &jobs, bool splitHeavy,
-                                       const SchedParams &sp)
+BrickSet make_brick_set(int vx, int vy, int vz, float depthScale)
+{
+  BrickSet set;
+  set.sorted = brick_candidates(vx, vy, vz, depthScale);
+  std::stable_sort(set.sorted.begin(), set.sorted.end(),
+                   [](const BrickCand &a, const BrickCand &b) { return a.est > b.est; });
+  for (const BrickCand &b : set.sorted) set.estSum += b.est;
+  return set;
+}
+
+void make_schedule(const std::vector<JobWork> &jobs, bool splitHeavy, const SchedParams &sp,
+                   std::vector<BrickOrigin> &bricks)
 {
   double total = 0;
   size_t count = 0;
   for (const JobWork &j : jobs) {
-    double sum = 0;
-    for (const BrickCand &c : *j.cands) sum += c.est;
-    total += sum * j.scale;
-    count += j.cands->size();
+    total += j.set->estSum * j.scale;
+    count += j.set->sorted.size();
   }
   // a wavefront's share of the work on a full MI355X (16 wavefronts on each of 256 CUs); only
   // bricks well above both that share and the typical brick are prioritised or split
   const double fair = total / 4096.0;
   const double mean = total / (double)std::max<size_t>(count, 1);
-  std::vector<BrickOrigin> bricks;
-  bricks.reserve(count + count / 8);
-  for (const JobWork &j : jobs) {
-    for (const BrickCand &c : *j.cands) {
+  const double at = fair * (double)sp.splitAt;
+  // An item's sort key: its float cost, then (job position, rank of the brick in the job's
+  // sorted bricks, part).  A job's bricks come sorted by cost and all scale by the same
+  // factor, so each job is a sorted stream; heavy bricks, the only ones prioritised or split,
+  // form a prefix of it.  Their parts (cost / parts) are sorted apart, then the streams are
+  // merged: a stable sort by cost of the jobs' sorted bricks in turn gives the same sequence.
+  struct Key {
+    float est;
+    int job;
+    uint32_t rank;
+    int part;
+  };
+  auto before = [](const Key &a, const Key &b) {
+    if (a.est != b.est) return a.est > b.est;
+    if (a.job != b.job) return a.job < b.job;
+    if (a.rank != b.rank) return a.rank < b.rank;
+    return a.part < b.part;
+  };
+  struct Item {
+    Key key;
+    BrickOrigin b;
+  };
+  std::vector<Item> heavy;
+  std::vector<size_t> pos(jobs.size(), 0);
+  for (size_t jj = 0; jj < jobs.size(); ++jj) {
+    const JobWork &j = jobs[jj];
+    const auto &sorted = j.set->sorted;
+    size_t p = 0;
+    for (; p < sorted.size(); ++p) {
+      const BrickCand &c = sorted[p];
       const double est = c.est * j.scale;
+      if (!(est > 4 * mean && (est > fair / 4 || est > at))) break;  // the rest is plain
       int flags = c.d;
-      if (sp.priority && est > fair / 4 && est > 4 * mean) flags |= 1 << 4;  // raised priority
+      if (sp.priority && est > fair / 4) flags |= 1 << 4;  // raised priority
       int parts = 1;
-      const double at = fair * (double)sp.splitAt;
-      if (splitHeavy && sp.maxParts > 1 && est > at && est > 4 * mean)
+      if (splitHeavy && sp.maxParts > 1 && est > at)
         parts = (int)std::min<double>(sp.maxParts, std::ceil(est / at));
       for (int k = 0; k < parts; ++k) {
-        BrickOrigin b{ c.x, c.y, c.z | (j.job << 16), flags | (parts << 8) | (k << 16), 0.f };
-        b.est = (float)(est / parts);
-        bricks.push_back(b);
+        BrickOrigin b{ c.x, c.y, c.z | (j.job << 16), flags | (parts << 8) | (k << 16),
+                       (float)(est / parts) };
+        heavy.push_back({ { b.est, (int)jj, (uint32_t)p, k }, b });
       }
     }
+    pos[jj] = p;
   }
-  // heaviest first, so the long central bricks start at once
-  std::stable_sort(bricks.begin(), bricks.end(),
-    [](const BrickOrigin &a, const BrickOrigin &b) { return a.est > b.est; });
-  return bricks;
+  std::sort(heavy.begin(), heavy.end(),
+            [&](const Item &a, const Item &b) { return before(a.key, b.key); });
+  bricks.clear();
+  bricks.reserve(count + heavy.size());
+  size_t h = 0;
+  auto emit_plain = [&](size_t jj, size_t p) {
+    const JobWork &j = jobs[jj];
+    const BrickCand &c = j.set->sorted[p];
+    bricks.push_back({ c.x, c.y, c.z | (j.job << 16), c.d | (1 << 8),
+                       (float)(c.est * j.scale) });
+  };
+  if (jobs.size() == 1) {
+    // one stream: merge the heavy parts into it
+    const size_t n = jobs[0].set->sorted.size();
+    const double scale = jobs[0].scale;
+    for (size_t p = pos[0]; p < n; ++p) {
+      const Key k{ (float)(jobs[0].set->sorted[p].est * scale), 0, (uint32_t)p, 0 };
+      while (h < heavy.size() && before(heavy[h].key, k)) bricks.push_back(heavy[h++].b);
+      emit_plain(0, p);
+    }
+    while (h < heavy.size()) bricks.push_back(heavy[h++].b);
+    return;
+  }
+  // k-way merge of the heavy items and the jobs' remaining streams.  A stream head's key packs
+  // into 64 bits, smallest first: the cost's float bits inverted (costs are positive, so their
+  // bits grow with them), the job position (8 bits) and the rank (24 bits).
+  auto head = [&](size_t jj) {
+    const JobWork &j = jobs[jj];
+    const float e = (float)(j.set->sorted[pos[jj]].est * j.scale);
+    uint32_t eb;
+    std::memcpy(&eb, &e, 4);
+    return ((uint64_t)~eb << 32) | ((uint64_t)jj << 24) | (uint64_t)pos[jj];
+  };
+  bool packable = jobs.size() <= 256;
+  for (const JobWork &j : jobs) packable = packable && j.set->sorted.size() < (1u << 24);
+  if (!packable) {
+    // beyond the packed key's ranges: one sort of every item by the full key
+    std::vector<Item> all;
+    all.reserve(count - pos.size() + heavy.size());
+    for (size_t jj = 0; jj < jobs.size(); ++jj)
+      for (size_t p = pos[jj]; p < jobs[jj].set->sorted.size(); ++p) {
+        const BrickCand &c = jobs[jj].set->sorted[p];
+        const float e = (float)(c.est * jobs[jj].scale);
+        all.push_back({ { e, (int)jj, (uint32_t)p, 0 },
+                        { c.x, c.y, c.z | (jobs[jj].job << 16), c.d | (1 << 8), e } });
+      }
+    all.insert(all.end(), heavy.begin(), heavy.end());
+    std::sort(all.begin(), all.end(),
+              [&](const Item &a, const Item &b) { return before(a.key, b.key); });
+    for (const Item &it : all) bricks.push_back(it.b);
+    return;
+  }
+  std::vector<uint64_t> q;
+  for (size_t jj = 0; jj < jobs.size(); ++jj)
+    if (pos[jj] < jobs[jj].set->sorted.size()) q.push_back(head(jj));
+  std::make_heap(q.begin(), q.end(), std::greater<uint64_t>());
+  while (!q.empty()) {
+    const uint64_t top = q.front();
+    const size_t jj = (size_t)((top >> 24) & 255u);
+    if (h < heavy.size()) {
+      uint32_t eb = ~(uint32_t)(top >> 32);
+      float e;
+      std::memcpy(&e, &eb, 4);
+      const Key k{ e, (int)jj, (uint32_t)(top & 0xffffffu), 0 };
+      while (h < heavy.size() && before(heavy[h].key, k)) bricks.push_back(heavy[h++].b);
+    }
+    std::pop_heap(q.begin(), q.end(), std::greater<uint64_t>());
+    emit_plain(jj, pos[jj]);
+    if (++pos[jj] < jobs[jj].set->sorted.size()) {
+      q.back() = head(jj);
+      std::push_heap(q.begin(), q.end(), std::greater<uint64_t>());
+    } else {
+      q.pop_back();
+    }
+  }
+  while (h < heavy.size()) bricks.push_back(heavy[h++].b);
+}
+
+std::vector<BrickOrigin> make_schedule(const std::vector<JobWork> &jobs, bool splitHeavy,
+                                       const SchedParams &sp)
+{
+  std::vector<BrickOrigin> out;
+  make_schedule(jobs, splitHeavy, sp, out);
+  return out;
 }
 
 std::vector<BrickOrigin> make_brick_schedule(int vx, int vy, int vz, bool splitHeavy,
                                              const SchedParams &sp)
 {
-  const std::vector<BrickCand> cands = brick_candidates(vx, vy, vz, sp.depthScale);
-  return make_schedule({ JobWork{ &cands, 0, 1.0 } }, splitHeavy, sp);
+  const BrickSet set = make_brick_set(vx, vy, vz, sp.depthScale);
+  return make_schedule({ JobWork{ &set, 0, 1.0 } }, splitHeavy, sp);
 }
 
 int symmetry_operators(char family, int n, double *out, uint32_t cap, uint32_t *count)
@@ -328,19 +464,84 @@ int symmetry_operators(char family, int n, double *out, uint32_t cap, uint32_t *
   return 0;
 }
 
+bool sched_params_valid(const umpa_fr_sched_params &p)
+{
+  return std::isfinite(p.depthScale) && p.depthScale > 0.f && p.depthScale <= 64.f &&
+         p.maxParts >= 1 && p.maxParts <= 255 && std::isfinite(p.splitAt) && p.splitAt > 0.f &&
+         p.splitAt <= 1024.f && p.kernel >= UMPA_FR_KERNEL_AUTO &&
+         p.kernel <= UMPA_FR_KERNEL_STAGED && (p.tableGlobal == 0 || p.tableGlobal == 1) &&
+         (p.boxFixed == 0 || p.boxFixed == 1);
+}
+
+SchedParams from_c(const umpa_fr_sched_params &p)
+{
+  SchedParams s;
+  s.depthScale = p.depthScale;
+  s.priority = p.priority != 0;
+  s.maxParts = p.maxParts;
+  s.splitAt = p.splitAt;
+  s.kernel = p.kernel;
+  s.tableGlobal = p.tableGlobal != 0;
+  s.boxFixed = p.boxFixed != 0;
+  return s;
+}
+
+umpa_fr_sched_params to_c(const SchedParams &p)
+{
+  umpa_fr_sched_params c{};
+  c.depthScale = p.depthScale;
+  c.priority = p.priority ? 1 : 0;
+  c.maxParts = p.maxParts;
+  c.splitAt = p.splitAt;
+  c.kernel = p.kernel;
+  c.tableGlobal = p.tableGlobal ? 1 : 0;
+  c.boxFixed = p.boxFixed ? 1 : 0;
+  return c;
+}
+
 std::vector<SchedParams> tuning_candidates(bool fastMath)
 {
   std::vector<SchedParams> out;
   for (float depth : { 0.5f, 1.f, 2.f })
     for (bool prio : { true, false }) {
+      SchedParams p;
+      p.depthScale = depth;
+      p.priority = prio;
       if (!fastMath) {
-        out.push_back({ depth, prio, 1, 0.5f });
+        // exact: the two kernels at 12 waves per CU, the table in LDS or global memory
+        p.maxParts = 1;
+        for (int kernel : { UMPA_FR_KERNEL_STAGED, UMPA_FR_KERNEL_PAIR })
+          for (bool global : { false, true }) {
+            p.kernel = kernel;
+            p.tableGlobal = global;
+            out.push_back(p);
+          }
         continue;
       }
       for (auto [parts, at] : { std::pair<int, float>{ 1, 0.5f }, { 4, 0.5f }, { 16, 0.5f },
-                                { 16, 0.25f } })
-        out.push_back({ depth, prio, parts, at });
+                                { 16, 0.25f } }) {
+        p.maxParts = parts;
+        p.splitAt = at;
+        out.push_back(p);
+      }
     }
+  return out;
+}
+
+std::vector<SchedParams> kernel_candidates(const SchedParams &base, bool fastMath)
+{
+  std::vector<SchedParams> out;
+  if (!fastMath) return out;
+  // fast math: per-pair (16 waves) or carried queues, sliding or fixed boxes, table placement
+  for (int kernel : { UMPA_FR_KERNEL_CARRY, UMPA_FR_KERNEL_PAIR })
+    for (bool fixed : { false, true })
+      for (bool global : { false, true }) {
+        SchedParams p = base;
+        p.kernel = kernel;
+        p.boxFixed = fixed;
+        p.tableGlobal = global;
+        out.push_back(p);
+      }
   return out;
 }
 
@@ -374,16 +575,13 @@ bool tuning_cache_lookup(const std::string &key, SchedParams &out)
   while (std::getline(f, line)) {
     std::istringstream is(line);
     std::string k;
-    SchedParams p;
-    int prio = 1;
+    umpa_fr_sched_params c{};
     double ms = 0;
     // entries outside the knobs' valid ranges (a damaged or hand-edited file) are ignored
-    if (is >> k >> p.depthScale >> prio >> p.maxParts >> p.splitAt >> ms && k == key &&
-        std::isfinite(p.depthScale) && p.depthScale > 0.f && p.depthScale <= 64.f &&
-        p.maxParts >= 1 && p.maxParts <= 255 && std::isfinite(p.splitAt) && p.splitAt > 0.f &&
-        p.splitAt <= 1024.f) {
-      p.priority = prio != 0;
-      out = p;
+    if (is >> k >> c.depthScale >> c.priority >> c.maxParts >> c.splitAt >> c.kernel >>
+          c.tableGlobal >> c.boxFixed >> ms &&
+        k == key && (c.priority == 0 || c.priority == 1) && sched_params_valid(c)) {
+      out = from_c(c);
       found = true;
     }
   }
@@ -395,7 +593,8 @@ void tuning_cache_store(const std::string &key, const SchedParams &p, double ms)
   std::lock_guard<std::mutex> lock(g_cacheMutex);
   std::ofstream f(tuning_file(), std::ios::app);
   if (f) f << key << ' ' << p.depthScale << ' ' << (p.priority ? 1 : 0) << ' ' << p.maxParts
-           << ' ' << p.splitAt << ' ' << ms << '\n';
+           << ' ' << p.splitAt << ' ' << p.kernel << ' ' << (p.tableGlobal ? 1 : 0) << ' '
+           << (p.boxFixed ? 1 : 0) << ' ' << ms << '\n';
 }
 
 }  // namespace umpa::host

@@ -293,7 +293,8 @@ __device__ __forceinline__ float candidate_weight(float d2, float r2, int nTable
 // test.  Rows are read whole; pixels past the slice end read 0 through the bounds-checked
 // buffer.
 // Returns false when the footprint is wider than NB.
-template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds>
+template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds,
+          bool kShiftable = true>
 __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
                                            __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
                                            float qx, float qy, float zSqr, float dw,
@@ -302,28 +303,51 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
   const umpa_fr_constants &c = a.c;
   const float r = c.cBlobRadius, r2 = r * r;
   const float inf = __builtin_inff();
-  int minX = (int)ceilf(qx - r), maxX = (int)floorf(qx + r);
-  int minY = (int)ceilf(qy - r), maxY = (int)floorf(qy + r);
+  int minX = (int)ceilf(qx - r);
+  int minY = (int)ceilf(qy - r);
   minX = minX < 0 ? 0 : minX;
   minY = minY < 0 ? 0 : minY;
-  maxX = maxX > a.xSize - 1 ? a.xSize - 1 : maxX;
-  maxY = maxY > a.ySize - 1 ? a.ySize - 1 : maxY;
-  const int nx = maxX - minX, ny = maxY - minY;
-  const bool fits = (nx < kNB) && (ny < kNB);
   constexpr int kPix = kHalf ? 4 : 8;
-  // (float)j == (float)minX + jj exactly for slice indices
-  const float fx0 = (float)minX, fy0 = (float)minY;
-
+  bool fits = true;
   float xd2[kNB], yz[kNB];
+  if (kShiftable && a.shiftBox) {
+    // The box slides inside the slice (slices of at least kNB x kNB pixels): it covers the
+    // reference's bounding box [minX, maxX] x [minY, maxY] whenever that has at most kNB
+    // columns and rows (the host sets shiftBox only where floor(2r) + 1 <= kNB), and every
+    // other candidate in it lies farther than r from q, so it weighs 0 (the table's zero
+    // entry; fast math: the clamp, or the last entry within 0.5 / cIDeltaSqrt of r^2), with no
+    // per-candidate masking.
+    minX = minX < a.xSize - kNB ? minX : a.xSize - kNB;
+    minY = minY < a.ySize - kNB ? minY : a.ySize - kNB;
+    const float fx0 = (float)minX, fy0 = (float)minY;
 #pragma unroll
-  for (int jj = 0; jj < kNB; ++jj) {
-    const float xD = qx - (fx0 + (float)jj);
-    xd2[jj] = (jj <= nx) ? xD * xD : inf;
-  }
+    for (int jj = 0; jj < kNB; ++jj) {
+      const float xD = qx - (fx0 + (float)jj);
+      xd2[jj] = xD * xD;
+    }
 #pragma unroll
-  for (int ii = 0; ii < kNB; ++ii) {
-    const float dy = qy - (fy0 + (float)ii);
-    yz[ii] = (ii <= ny) ? dy * dy + zSqr : inf;
+    for (int ii = 0; ii < kNB; ++ii) {
+      const float dy = qy - (fy0 + (float)ii);
+      yz[ii] = dy * dy + zSqr;
+    }
+  } else {
+    int maxX = (int)floorf(qx + r), maxY = (int)floorf(qy + r);
+    maxX = maxX > a.xSize - 1 ? a.xSize - 1 : maxX;
+    maxY = maxY > a.ySize - 1 ? a.ySize - 1 : maxY;
+    const int nx = maxX - minX, ny = maxY - minY;
+    fits = (nx < kNB) && (ny < kNB);
+    // (float)j == (float)minX + jj exactly for slice indices
+    const float fx0 = (float)minX, fy0 = (float)minY;
+#pragma unroll
+    for (int jj = 0; jj < kNB; ++jj) {
+      const float xD = qx - (fx0 + (float)jj);
+      xd2[jj] = (jj <= nx) ? xD * xD : inf;
+    }
+#pragma unroll
+    for (int ii = 0; ii < kNB; ++ii) {
+      const float dy = qy - (fy0 + (float)ii);
+      yz[ii] = (ii <= ny) ? dy * dy + zSqr : inf;
+    }
   }
   const uint32_t rowStride = (uint32_t)(a.xSize * kPix);
   // sliceOff: the slice's byte offset within the buffer resource (0 when the resource covers
@@ -436,7 +460,8 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
 template <bool kFma>
 __device__ __forceinline__ bool blob_predicate(const umpa_traverse_space &s, const umpa_fr_constants &c,
                                                int x, int y, int z, float px, float py, float pz,
-                                               float dsq, float &qx, float &qy, float &zSqr)
+                                               bool brickInside, float &qx, float &qy,
+                                               float &zSqr)
 {
   const float r = c.cBlobRadius;
   const float(*t)[3] = s.transformInv;
@@ -452,15 +477,20 @@ __device__ __forceinline__ bool blob_predicate(const umpa_traverse_space &s, con
   }
   zSqr = qz * qz;
   // & rather than &&: every test is evaluated, so the record's fields load together
-  bool pass = !(dsq > s.maxDistanceSqr) & !(qx < -r) & !(zSqr > r * r);
+  bool pass = !(qx < -r) & !(zSqr > r * r);
   qy += (float)(c.cMaxVolumeIndexYZ / 2);
-  // traversal: the column must lie in the AABB's plane orthogonal to `dir`
-  if (s.dir == 0) {
-    pass = pass & (x >= s.minX) & (x <= s.maxX) & (y >= s.minY) & (y <= s.maxY);
-  } else if (s.dir == 1) {
-    pass = pass & (x >= s.minX) & (x <= s.maxX) & (z >= s.minZ) & (z <= s.maxZ);
-  } else {
-    pass = pass & (y >= s.minY) & (y <= s.maxY) & (z >= s.minZ) & (z <= s.maxZ);
+  if (!brickInside) {
+    // the sphere test, and the traversal: the column must lie in the AABB's plane orthogonal
+    // to `dir` (both hold for every voxel of a brick that lies inside them: brick_inside)
+    const float dsq = px * px + py * py + pz * pz;
+    pass = pass & !(dsq > s.maxDistanceSqr);
+    if (s.dir == 0) {
+      pass = pass & (x >= s.minX) & (x <= s.maxX) & (y >= s.minY) & (y <= s.maxY);
+    } else if (s.dir == 1) {
+      pass = pass & (x >= s.minX) & (x <= s.maxX) & (z >= s.minZ) & (z <= s.maxZ);
+    } else {
+      pass = pass & (y >= s.minY) & (y <= s.maxY) & (z >= s.minZ) & (z <= s.maxZ);
+    }
   }
   // ... and the voxel must lie in the column's slab between the two bounding planes.  A voxel
   // whose |qz| is at least 0.001 inside the blob radius is between the planes by at least that
@@ -558,8 +588,28 @@ __device__ __forceinline__ BrickCtx make_brick(const FrJob &a, const int4 &bo, i
   return b;
 }
 
+// Do all of the brick's voxels pass the reference's sphere test (|p|^2 <= maxDistanceSqr; the
+// farthest corner's squared distance is an exact integer in float, as is every voxel's) and lie
+// in the space's traversal plane (the AABB extents orthogonal to `dir`)?  Wave-uniform.
+__device__ __forceinline__ bool brick_inside(const BrickCtx &b, const umpa_traverse_space &sp,
+                                             int cxi, int cyi)
+{
+  const int x1 = b.x0 + kEdge - 1, y1 = b.y0 + kEdge - 1, z1 = b.z0 + b.nz - 1;
+  const int mx = max(abs(b.x0 - cxi), abs(x1 - cxi));
+  const int my = max(abs(b.y0 - cyi), abs(y1 - cyi));
+  const int mz = max(abs(b.z0 - cyi), abs(z1 - cyi));
+  const bool sphere = (float)(mx * mx + my * my + mz * mz) <= sp.maxDistanceSqr;
+  const bool inX = b.x0 >= sp.minX && x1 <= sp.maxX;
+  const bool inY = b.y0 >= sp.minY && y1 <= sp.maxY;
+  const bool inZ = b.z0 >= sp.minZ && z1 <= sp.maxZ;
+  const int dir = sp.dir;
+  const bool box = dir == 0 ? (inX && inY) : dir == 1 ? (inX && inZ) : (inY && inZ);
+  return __builtin_amdgcn_readfirstlane((int)(sphere && box)) != 0;
+}
+
 // The brick's running sums start from the volume (or from zero for a split part).
-__device__ __forceinline__ void load_brick(WaveScratch &ws, const FrJob &a, const BrickCtx &b,
+template <typename Scratch>
+__device__ __forceinline__ void load_brick(Scratch &ws, const FrJob &a, const BrickCtx &b,
                                            int lane, bool fromZero)
 {
   for (int k = 0; k < b.nz; ++k) {
@@ -580,7 +630,8 @@ __device__ __forceinline__ void load_brick(WaveScratch &ws, const FrJob &a, cons
 }
 
 // The brick's sums back to the volume: stored, or added with atomics for a split part.
-__device__ __forceinline__ void store_brick(const WaveScratch &ws, const FrJob &a,
+template <typename Scratch>
+__device__ __forceinline__ void store_brick(const Scratch &ws, const FrJob &a,
                                             const BrickCtx &b, int lane, bool split)
 {
   for (int k = 0; k < b.nz; ++k) {
@@ -739,6 +790,7 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
         if (cnt == 0) continue;
         wave_sync();
         const bool unit = sp.weight == 1.f;
+        const bool inside = brick_inside(bk, sp, cxi, cyi);
 
         // phase 2: the rest of the predicate and the gather, with full lanes
         for (int t0 = 0; t0 < cnt; t0 += 64) {
@@ -748,16 +800,16 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
           const int vl = v & 63, vk = v >> 6;
           const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bz + vk;
           const float pxv = (float)(xv - cxi), pyv = (float)(yv - cyi), pzv = (float)(zv - cyi);
-          const float dsq = pxv * pxv + pyv * pyv + pzv * pzv;
           float qx, qy, zs;
           bool hit;
           if constexpr (kMode == kModeNearest) {
+            const float dsq = pxv * pxv + pyv * pyv + pzv * pzv;
             qx = sp.transformInv[0][0] * pxv + sp.transformInv[0][1] * pyv + sp.transformInv[0][2] * pzv;
             qy = sp.transformInv[1][0] * pxv + sp.transformInv[1][1] * pyv + sp.transformInv[1][2] * pzv;
             zs = 0.f;
             hit = act && !(dsq > sp.maxDistanceSqr) && !(qx < 0.f) && visited_nearest(sp, xv, yv, zv);
           } else {
-            hit = act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, dsq, qx, qy, zs);
+            hit = act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, inside, qx, qy, zs);
           }
           if (hit) {
             Contribution o;
@@ -781,10 +833,10 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
             } else {
               bool fits;
               if (unit) {
-                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds>(
+                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds, kFastMath>(
                   a, nTable, rsrc, tbl, qx, qy, zs, sp.weight, o);
               } else {
-                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds>(
+                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds, kFastMath>(
                   a, nTable, rsrc, tbl, qx, qy, zs, sp.weight, o);
               }
               if (!fits)
@@ -803,6 +855,237 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
         }
         wave_sync();
       }
+    }
+    store_brick(ws, a, bk, lane, split);
+    if (heavy) __builtin_amdgcn_s_setprio(0);
+    if constexpr (kCount) {
+      if (lane == 0 && nupd) atomicAdd(l.updates, nupd);
+    }
+  }
+}
+
+// ---- fast-math lookup insertion with queues carried across spaces ---------------------------
+//
+// fr_insert_kernel gathers each (brick, space) queue 64 voxels at a time, so the last batch of
+// every pair runs partly empty.  This variant (fast math, FP32 or FP16 slices, table in LDS,
+// 16 wavefronts per CU) lets the slab voxels of consecutive spaces share one queue: a space's
+// voxels are appended behind the ones left from the previous spaces, every full batch of 64 is
+// gathered as soon as it forms, and a partial batch runs only at the end of a brick or when a
+// third space would be pending.  An entry carries the slot (0 / 1) of its space's record, so a
+// batch holds at most two spaces; it adds their contributions space by space, the earlier one
+// first (no voxel appears twice within one space), so every voxel still receives its spaces in
+// index order and the sums are the ones fr_insert_kernel forms.  All slices of the chunk are
+// read through one buffer resource (chunks up to 2 GiB of slices).
+constexpr int kCarryQueue = kVox + 64;  // leftovers (< 64) plus one space's slab voxels
+
+struct CarrySlot {
+  umpa_traverse_space rec;
+  uint32_t sliceOff;  // byte offset of the space's slice in the chunk
+  float scale;        // FP16 dequantisation factor of that slice
+  int inside;         // brick_inside() of the brick and this space
+  uint32_t pad;
+};
+static_assert(sizeof(CarrySlot) == 128, "two slots per wave");
+
+struct CarryScratch {
+  float accR[kVox];
+  float accI[kVox];
+  float accW[kVox];
+  uint16_t queue[kCarryQueue];  // voxel index (bits 0-8) | slot (bit 9)
+};
+
+// Largest blob table the carry kernel keeps in LDS beside its per-wave state.
+constexpr int kCarryMaxTable =
+  (160 * 1024 - kMaxWaves * (int)(sizeof(CarryScratch) + 2 * sizeof(CarrySlot))) / 4 - 1;
+
+// Slab test of the brick's voxels against `sp`, appended to the queue behind `pending`
+// entries with the space's slot in bit 9; returns the number appended.
+__device__ __forceinline__ int slab_queue_carry(CarryScratch &ws, const FrJob &a,
+                                                const BrickCtx &b, const umpa_traverse_space &sp,
+                                                float slab2, int lane, int pending, int slot)
+{
+  const int cyi = a.c.cMaxVolumeIndexYZ / 2;
+  const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
+              t22 = sp.transformInv[2][2];
+  const float cz = t20 * b.px + t21 * b.py;
+  int cnt = 0;
+  const uint32_t tag = (uint32_t)slot << 9;
+  for (int k = 0; k < b.nz; ++k) {
+    const float pz = (float)(b.z0 + k - cyi);
+    const float qz = cz + t22 * pz;
+    const bool ok = b.colIn && !(qz * qz > slab2);
+    const unsigned long long m = __ballot(ok);
+    if (ok) ws.queue[pending + cnt + (int)lanes_below(m)] = (uint16_t)((uint32_t)(k * 64 + lane) | tag);
+    cnt += __popcll(m);
+  }
+  return cnt;
+}
+
+// One batch of `n` (1..64) queue entries from `start`: the rest of the predicate, the gather and
+// the accumulation, space by space.
+template <int kNB, bool kHalf, bool kCount>
+__device__ __forceinline__ void carry_batch(CarryScratch &ws, const CarrySlot *slots,
+                                            const FrJob &a, const int4 &bo, int bz, int nTable,
+                                            __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
+                                            int lane, int start, int n,
+                                            unsigned long long &nupd)
+{
+  const umpa_fr_constants &c = a.c;
+  const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
+  const bool act = lane < n;
+  const uint32_t e = act ? (uint32_t)ws.queue[start + lane] : 0u;
+  const int v = (int)(e & 511u), tag = (int)(e >> 9);
+  const CarrySlot &slt = slots[tag];
+  const umpa_traverse_space &sp = slt.rec;
+  const int vl = v & 63, vk = v >> 6;
+  const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bz + vk;
+  const float pxv = (float)(xv - cxi), pyv = (float)(yv - cyi), pzv = (float)(zv - cyi);
+  float qx, qy, zs;
+  const bool hit =
+    act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, slt.inside != 0, qx, qy, zs);
+  Contribution o{ 0.f, 0.f, 0.f };
+  if (hit) {
+    const uint32_t sliceOff = slt.sliceOff;
+    const float dw = sp.weight;
+    bool fits;
+    if (__all(!hit || dw == 1.f)) {
+      fits = gather_box<kModeLookup, 0, kNB, kHalf, true, true, true>(a, nTable, rsrc, tbl, qx,
+                                                                       qy, zs, dw, o, sliceOff);
+    } else {
+      fits = gather_box<kModeLookup, 0, kNB, kHalf, true, false, true>(a, nTable, rsrc, tbl, qx,
+                                                                        qy, zs, dw, o, sliceOff);
+    }
+    if (!fits)
+      o = gather_generic<kModeLookup, 0, kHalf, true>(a, nTable, rsrc, tbl, qx, qy, zs, dw,
+                                                     sliceOff);
+    if constexpr (kHalf) {
+      o.re *= slt.scale;
+      o.im *= slt.scale;
+    }
+  }
+  // lane 0 holds the batch's oldest entry: its space is added first
+  const int first = __builtin_amdgcn_readfirstlane(tag);
+  if (hit && tag == first) {
+    ws.accR[v] = ws.accR[v] + o.re;
+    ws.accI[v] = ws.accI[v] + o.im;
+    ws.accW[v] = ws.accW[v] + o.w;
+  }
+  if (__any(hit && tag != first)) {
+    wave_sync();
+    if (hit && tag != first) {
+      ws.accR[v] = ws.accR[v] + o.re;
+      ws.accI[v] = ws.accI[v] + o.im;
+      ws.accW[v] = ws.accW[v] + o.w;
+    }
+  }
+  if constexpr (kCount) nupd += (unsigned long long)__popcll(__ballot(hit));
+}
+
+template <int kNB, bool kHalf, bool kCount, bool kBatched>
+__global__ __launch_bounds__(64 * kMaxWaves) void fr_insert_carry_kernel(FrJob job0, FrLaunch l)
+{
+  constexpr int kWaves = kMaxWaves;
+  constexpr int kThreads = 64 * kWaves;
+  constexpr int kPix = kHalf ? 4 : 8;
+  extern __shared__ float s_table[];
+  __shared__ CarryScratch s_wave[kWaves];
+  __shared__ CarrySlot s_slot[kWaves][2];
+  const int nTable = l.nTable;
+  for (int i = threadIdx.x; i < nTable; i += kThreads) s_table[i] = l.table[i];
+  if (threadIdx.x == 0) s_table[nTable] = 0.f;
+  __syncthreads();
+  const float *tbl = s_table;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  CarryScratch &ws = s_wave[wv];
+  CarrySlot *slots = s_slot[wv];
+
+  for (;;) {
+    int b = 0;
+    if (lane == 0) b = atomicAdd(l.counter, 1);
+    b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
+    if (b >= l.nItems) break;
+    const int4 bo = make_uniform(reinterpret_cast<const int4 *>(l.items)[b]);
+    const FrJob a = kBatched ? make_uniform(l.jobs[bo.z >> 16]) : job0;
+    const int s0 = a.spaceBegin, s1 = a.spaceEnd;
+    if (s0 >= s1) continue;
+    const bool heavy = (bo.w >> 4) & 1;
+    const int parts = (bo.w >> 8) & 255, part = (bo.w >> 16) & 255;
+    const bool split = parts > 1;
+    const int sBegin = split ? s0 + (int)((int64_t)(s1 - s0) * part / parts) : s0;
+    const int sEnd = split ? s0 + (int)((int64_t)(s1 - s0) * (part + 1) / parts) : s1;
+    const float r = a.c.cBlobRadius;
+    const float slab2 = r * r;
+    const int cxi = a.c.cMaxVolumeIndexX / 2, cyi = a.c.cMaxVolumeIndexYZ / 2;
+    if (heavy) __builtin_amdgcn_s_setprio(2);
+    const BrickCtx bk = make_brick(a, bo, lane);
+    const int bz = bk.z0;
+    load_brick(ws, a, bk, lane, split);
+    const uint32_t sliceBytes = (uint32_t)(a.sliceElems * kPix);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)a.fft, 0, (int)((uint32_t)a.nProj * sliceBytes), kBufferConfig);
+    const bool earlyCull = __builtin_amdgcn_readfirstlane((int)(bk.dmin > kEarlyCullRadius)) != 0;
+    unsigned long long nupd = 0;
+    int pending = 0;     // queue entries waiting, queue[0 .. pending)
+    int pendSpaces = 0;  // spaces they come from (at most 2)
+    int slot = 0;        // record slot of the next space
+    for (int base = sBegin; base < sEnd; base += 64) {
+      unsigned long long mask = earlyCull
+                                  ? __ballot(cull_space<true>(a, bk, base + lane, sEnd))
+                                  : __ballot(cull_space<false>(a, bk, base + lane, sEnd));
+      constexpr int kRecDwords = (int)(sizeof(umpa_traverse_space) / 4);
+      const bool recLane = lane < kRecDwords;
+      int rec = 0;
+      if (mask && recLane)
+        rec = reinterpret_cast<const int *>(
+          a.spaces + __builtin_amdgcn_readfirstlane(base + __builtin_ctzll(mask)))[lane];
+      while (mask) {
+        mask &= mask - 1;
+        if (pendSpaces == 2) {
+          // the slot about to be rewritten still holds a pending space: gather everything
+          wave_sync();
+          carry_batch<kNB, kHalf, kCount>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane, 0,
+                                          pending, nupd);
+          pending = 0;
+          pendSpaces = 0;
+          wave_sync();
+        }
+        const int proj = __builtin_amdgcn_readlane(rec, 17);  // projectionIndex (byte 68)
+        if (recLane) reinterpret_cast<int *>(&slots[slot])[lane] = rec;
+        if (lane == kRecDwords) slots[slot].sliceOff = (uint32_t)proj * sliceBytes;
+        if (lane == kRecDwords + 1) slots[slot].scale = kHalf ? a.scales[proj] : 1.f;
+        wave_sync();
+        if (mask && recLane)
+          rec = reinterpret_cast<const int *>(
+            a.spaces + __builtin_amdgcn_readfirstlane(base + __builtin_ctzll(mask)))[lane];
+        const bool ins = brick_inside(bk, slots[slot].rec, cxi, cyi);
+        if (lane == 0) slots[slot].inside = ins ? 1 : 0;
+        const int cnt = slab_queue_carry(ws, a, bk, slots[slot].rec, slab2, lane, pending, slot);
+        if (cnt == 0) continue;  // the slot stays free for the next space
+        pending += cnt;
+        ++pendSpaces;
+        slot ^= 1;
+        wave_sync();
+        const int full = pending >> 6;
+        if (full) {
+          for (int t = 0; t < full; ++t)
+            carry_batch<kNB, kHalf, kCount>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane,
+                                            t * 64, 64, nupd);
+          // the leftovers (< 64, all of the space just queued) move to the queue's front
+          const int left = pending - (full << 6);
+          const int x = lane < left ? (int)ws.queue[(full << 6) + lane] : 0;
+          wave_sync();
+          if (lane < left) ws.queue[lane] = (uint16_t)x;
+          pending = left;
+          pendSpaces = left ? 1 : 0;
+        }
+        wave_sync();
+      }
+    }
+    if (pending) {
+      carry_batch<kNB, kHalf, kCount>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane, 0, pending,
+                                      nupd);
+      wave_sync();
     }
     store_brick(ws, a, bk, lane, split);
     if (heavy) __builtin_amdgcn_s_setprio(0);
@@ -851,10 +1134,10 @@ __device__ __forceinline__ void staged_gather(const FrJob &a, int nTable,
     // reference's, and wb * 1 = wb, so the choice per batch changes no bit
     bool fits;
     if (__all(dw == 1.f)) {
-      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true, false>(a, nTable, rsrc, tbl, qx,
                                                                        qy, zs, dw, o, off);
     } else {
-      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true, false>(a, nTable, rsrc, tbl, qx,
                                                                         qy, zs, dw, o, off);
     }
     if (!fits)
@@ -936,6 +1219,7 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
         const int cnt = slab_queue(ws, a, bk, sp, r2, lane);
         if (cnt == 0) continue;
         wave_sync();
+        const bool inside = brick_inside(bk, sp, cxi, cyi);
         // phase 2: the rest of the predicate; hits go to the ring, gathered 64 at a time
         for (int t0 = 0; t0 < cnt; t0 += 64) {
           const int item = t0 + lane;
@@ -944,10 +1228,9 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
           const int vl = v & 63, vk = v >> 6;
           const int xv = bo.x + (vl & (kEdge - 1)), yv = bo.y + vl / kEdge, zv = bz + vk;
           const float pxv = (float)(xv - cxi), pyv = (float)(yv - cyi), pzv = (float)(zv - cyi);
-          const float dsq = pxv * pxv + pyv * pyv + pzv * pzv;
           float qx, qy, zs;
           const bool hit =
-            act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, dsq, qx, qy, zs);
+            act && blob_predicate<false>(sp, c, xv, yv, zv, pxv, pyv, pzv, inside, qx, qy, zs);
           const unsigned long long hm = __ballot(hit);
           if (hit) {
             const int e = (head + waiting + (int)lanes_below(hm)) & (kStage - 1);
@@ -1027,7 +1310,7 @@ static const void *pick_order(int order, bool batched)
 
 static bool table_in_lds(const FrVariant &v, int nTable)
 {
-  return v.mode == kModeLookup && nTable <= kMaxLdsTable;
+  return v.mode == kModeLookup && nTable <= kMaxLdsTable && !(v.tableGlobal && !v.batched);
 }
 
 static const void *pick_kernel(const FrVariant &v, int nTable)
@@ -1073,6 +1356,31 @@ static bool staged_fits(const FrJob &job)
   return (uint64_t)job.nProj * (uint64_t)job.sliceElems * 8u <= 0x7fff0000ull;
 }
 
+// The fast-math kernel with queues carried across spaces, for the variants it covers (the
+// launch also needs the chunk's slices under 2 GiB, one buffer resource).
+static const void *pick_carry(const FrVariant &v, int nTable)
+{
+  if (v.mode != kModeLookup || !v.fastMath || v.batched) return nullptr;
+  if (!table_in_lds(v, nTable) || nTable > kCarryMaxTable) return nullptr;
+  if (v.half) {
+    if (v.nb == 3) return v.count ? (const void *)fr_insert_carry_kernel<3, true, true, false>
+                                  : (const void *)fr_insert_carry_kernel<3, true, false, false>;
+    if (v.nb == 4) return v.count ? (const void *)fr_insert_carry_kernel<4, true, true, false>
+                                  : (const void *)fr_insert_carry_kernel<4, true, false, false>;
+    return nullptr;
+  }
+  if (v.nb == 3) return v.count ? (const void *)fr_insert_carry_kernel<3, false, true, false>
+                                : (const void *)fr_insert_carry_kernel<3, false, false, false>;
+  if (v.nb == 4) return v.count ? (const void *)fr_insert_carry_kernel<4, false, true, false>
+                                : (const void *)fr_insert_carry_kernel<4, false, false, false>;
+  return nullptr;
+}
+
+static bool carry_fits(const FrJob &job, bool half)
+{
+  return (uint64_t)job.nProj * (uint64_t)job.sliceElems * (half ? 4u : 8u) <= 0x7fff0000ull;
+}
+
 hipError_t prepare_fr_insert(const FrVariant &v, int nTable)
 {
   const void *k = pick_kernel(v, nTable);
@@ -1083,6 +1391,9 @@ hipError_t prepare_fr_insert(const FrVariant &v, int nTable)
   const void *staged = pick_staged(v, nTable);
   if (e == hipSuccess && staged)
     e = hipFuncSetAttribute(staged, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  const void *carry = pick_carry(v, nTable);
+  if (e == hipSuccess && carry)
+    e = hipFuncSetAttribute(carry, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   return e;
 }
 
@@ -1094,10 +1405,20 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
   const size_t shmem = table_in_lds(v, l.nTable) ? (size_t)(l.nTable + 1) * sizeof(float) : 0;
   // persistent workgroups: the per-wave LDS state allows one per CU
   int waves = fr_waves(v.mode, v.half, v.fastMath);
-  const void *staged = pick_staged(v, l.nTable);
+  // kKernelAuto: the staged kernel for the exact strategy, the carried queues for fast-math
+  // FP32 slices (FP16 slices run faster on the per-pair kernel: 1.22 vs 1.26 ms per 1,000-slice
+  // chunk at box 256)
+  const int want = v.kernel != kKernelAuto ? v.kernel
+                   : !v.fastMath          ? kKernelStaged
+                   : v.half               ? kKernelPair
+                                          : kKernelCarry;
+  const void *staged = want == kKernelStaged ? pick_staged(v, l.nTable) : nullptr;
+  const void *carry = want == kKernelCarry ? pick_carry(v, l.nTable) : nullptr;
   if (staged && staged_fits(job)) {
     k = staged;
     waves = kStagedWaves;
+  } else if (carry && carry_fits(job, v.half)) {
+    k = carry;
   }
   int blocks = (l.nItems + waves - 1) / waves;
   if (blocks > numCUs) blocks = numCUs;

@@ -24,7 +24,7 @@ struct FrJob {
   int vx, vy, vz;       // volume extents
   int spaceBegin;       // this launch inserts spaces [spaceBegin, spaceEnd)
   int spaceEnd;
-  int pad0;
+  int shiftBox;         // the candidate box may slide inside the slice (see gather_box)
   umpa_fr_constants c;
   float cullHalfWidth;  // slab half width used by the brick cull (blob radius, 1.6 for nearest)
   float cullHalfSpace;  // allowed q.x below zero (blob radius, 0 for nearest)
@@ -56,7 +56,15 @@ struct FrVariant {
   bool fastMath;  // fused multiply-add accumulation
   bool count;     // count voxel-updates
   bool batched;   // jobs from FrLaunch::jobs (else the FrJob argument)
+  int kernel;     // kKernel*: which insertion kernel (kKernelAuto: the measured best)
+  bool tableGlobal;  // read the blob table from global memory even where LDS holds it
 };
+
+// Insertion kernels: one queue per (brick, space) pair; queues carried across spaces (fast
+// math); a cross-space ring of staged entries (exact FP32 lookup, 12 wavefronts per CU).
+// A kernel that does not cover a variant (mode, strategy, footprint, batching) or a chunk
+// (slices beyond one 2 GiB buffer resource) is replaced by the per-pair kernel.
+enum : int { kKernelAuto = 0, kKernelPair = 1, kKernelCarry = 2, kKernelStaged = 3 };
 
 // Dynamic LDS and workgroup count of an insertion launch; `prepare` sets the kernel's LDS
 // attribute once (outside any stream capture).

@@ -16,12 +16,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import (FFT_F32, FFT_F64, FFT_FORWARD, FFT_INVERSE, FR_DYNAMIC, FR_EXACT,
-                   FR_FAST, FR_FAST_MATH, FR_LOOKUP, FR_PRECISE, TRAVERSE_SPACE_DTYPE, FFTDesc,
+                   FR_FAST, FR_FAST_MATH, FR_KERNEL_AUTO, FR_KERNEL_CARRY, FR_KERNEL_PAIR,
+                   FR_KERNEL_STAGED, FR_LOOKUP, FR_PRECISE, TRAVERSE_SPACE_DTYPE, FFTDesc,
                    FPDesc, FPSettings, FRConstants, FRDesc, FRJob, FRSettings, SchedParams, Size,
                    check)
 
 __all__ = [
     "FR_DYNAMIC", "FR_LOOKUP", "FR_FAST", "FR_PRECISE", "FR_EXACT", "FR_FAST_MATH",
+    "FR_KERNEL_AUTO", "FR_KERNEL_PAIR", "FR_KERNEL_CARRY", "FR_KERNEL_STAGED",
+    "tuning_candidates",
     "TRAVERSE_SPACE_DTYPE", "FRSettings", "FRDesc", "FPSettings", "Size",
     "FourierReconstruction", "FourierProcessing", "create_constants", "fill_blob_table",
     "compute_traverse_space", "traverse_spaces", "generate_traverse_spaces", "BatchJob",
@@ -317,6 +320,18 @@ def brick_schedule(vx: int, vy: int, vz: int, split_heavy: bool,
                                      out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)),
           "umpa_fr_brick_schedule")
     return out
+
+
+def tuning_candidates(strategy: int) -> list:
+    """The knobs the autotuner times under `strategy` (umpa_fr_tuning_candidates)."""
+    lib = _lib.load()
+    n = C.c_uint64(0)
+    check(lib.umpa_fr_tuning_candidates(strategy, None, 0, C.byref(n)),
+          "umpa_fr_tuning_candidates")
+    arr = (SchedParams * n.value)()
+    check(lib.umpa_fr_tuning_candidates(strategy, arr, n.value, C.byref(n)),
+          "umpa_fr_tuning_candidates")
+    return list(arr)
 
 
 def check_batch_jobs(jobs, strategy: int = FR_EXACT) -> int:
