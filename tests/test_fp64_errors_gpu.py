@@ -6,10 +6,15 @@ in double), stated per workload on sub-problems the oracle finishes in seconds:
   FP16, serving-like: r 1.5 / 2.5, C4 / D2 symmetry   box 96
   FP16, large symmetric: D7 (14 operators)            box 256, 12 slices
 
-FP16 errors are measured against the FP64 oracle on the dequantized slices (the precision of
-the accumulation); the quantization itself is stated separately in the FP16 test of
-test_fr_gpu.py.  Bounds: 1e-4 relative L2 for every FP16 path (the stated FP16 tolerance
-against FP64) and for FP32."""
+FP16 errors are measured twice: against the FP64 oracle on the dequantized slices (the
+precision of the accumulation; bound 1e-4 relative L2, as for FP32), and end to end, against
+the FP64 oracle on the original FP32 slices.  End to end the FP16 storage dominates: with
+per-slice max-abs scaling every element is rounded to 11 significant bits, an error uniform in
++-half an ulp, rms 2^-11 / sqrt(3) = 2.8e-4 of 2^e for a value in [2^e, 2^(e+1)), i.e. 1.4e-4 to
+2.8e-4 of the value; the volume, weighted sums of independent slice values, carries about that
+relative error (1.3e-4 to 1.7e-4 measured on these inputs with the FP64 oracle alone), so 1e-4
+end to end is out of reach for FP16 storage; the bound asserted is 4e-4 on the volume.
+Weights depend on the geometry only and stay within 1e-4."""
 import numpy as np
 import pytest
 
@@ -21,10 +26,23 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def errors(s, fft_host, spaces, desc, table, box, half=None, strategy=F.FR_EXACT):
+def errors(s, fft_host, spaces, desc, table, box, half=None, strategy=F.FR_EXACT,
+           original=None):
+    """(volume, weights) relative L2 against the FP64 oracle on `fft_host`; with `original`
+    (the FP32 slices before quantization) also the volume's end-to-end error."""
     v, w, _ = T.run_hip(s, fft_host, spaces, desc, table, box, strategy=strategy, half=half)
     rv, rw = T.run_oracle_f64(s, fft_host, spaces, table, box)
-    return T.rel_l2(v, rv), T.rel_l2(w, rw)
+    if original is None:
+        return T.rel_l2(v, rv), T.rel_l2(w, rw)
+    ov, ow = T.run_oracle_f64(s, original, spaces, table, box)
+    return T.rel_l2(v, rv), T.rel_l2(w, rw), T.rel_l2(v, ov), T.rel_l2(w, ow)
+
+
+def check_fp16(label, ev, ew, e2e_v, e2e_w):
+    print(f"{label} FP16: volume {ev:.2e} (dequantized input), {e2e_v:.2e} end to end; "
+          f"weights {ew:.2e} / {e2e_w:.2e}")
+    assert ev <= 1e-4 and ew <= 1e-4
+    assert e2e_v <= 4e-4 and e2e_w <= 1e-4
 
 
 @pytest.mark.parametrize("strategy", [F.FR_EXACT, F.FR_FAST_MATH])
@@ -40,9 +58,7 @@ def test_box256_fp16_vs_fp64():
     s = T.settings()
     fft, spaces, desc, table = T.problem(256, 40, s, seed=61)
     h, sc, deq = T.quantize_f16(fft)
-    ev, ew = errors(s, deq, spaces, desc, table, 256, half=(h, sc))
-    print(f"box 256 FP16: volume {ev:.2e}, weights {ew:.2e}")
-    assert ev <= 1e-4 and ew <= 1e-4
+    check_fp16("box 256", *errors(s, deq, spaces, desc, table, 256, half=(h, sc), original=fft))
 
 
 @pytest.mark.parametrize("radius,group", [(1.5, "C4"), (2.5, "D2")])
@@ -52,9 +68,8 @@ def test_serving_like_fp16_vs_fp64(radius, group):
     fft, spaces, desc, table = T.problem(box, n, s, seed=71,
                                          symmetries=F.symmetry_operators(group))
     h, sc, deq = T.quantize_f16(fft)
-    ev, ew = errors(s, deq, spaces, desc, table, box, half=(h, sc))
-    print(f"box {box} r {radius} {group} FP16: volume {ev:.2e}, weights {ew:.2e}")
-    assert ev <= 1e-4 and ew <= 1e-4
+    check_fp16(f"box {box} r {radius} {group}",
+               *errors(s, deq, spaces, desc, table, box, half=(h, sc), original=fft))
 
 
 def test_d7_fp16_vs_fp64():
@@ -64,6 +79,5 @@ def test_d7_fp16_vs_fp64():
                                          symmetries=F.symmetry_operators("D7"))
     assert len(spaces) == 14 * n
     h, sc, deq = T.quantize_f16(fft)
-    ev, ew = errors(s, deq, spaces, desc, table, box, half=(h, sc))
-    print(f"box {box} D7 FP16: volume {ev:.2e}, weights {ew:.2e}")
-    assert ev <= 1e-4 and ew <= 1e-4
+    check_fp16(f"box {box} D7", *errors(s, deq, spaces, desc, table, box, half=(h, sc),
+                                         original=fft))
