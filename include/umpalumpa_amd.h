@@ -222,6 +222,12 @@ int umpa_fr_batch_num_steps(umpa_fr_batch *b, uint32_t *steps);
 /* counting variant for the next execute / step (bypasses the graph); result after it */
 int umpa_fr_batch_set_count_updates(umpa_fr_batch *b, int enable);
 int umpa_fr_batch_last_updates(umpa_fr_batch *b, uint64_t *updates);
+/* Bounded launches: with itemsPerWave > 0 every wavefront of the batch's later launches (and
+ * plans) leaves after that many work items, and the grid covers all items, so CUs come free
+ * while a launch runs and another stream's launches (a second batch) start at once instead of
+ * after it; 0 (the default) keeps one persistent workgroup per CU.  Results do not change. */
+int umpa_fr_batch_set_wave_items(umpa_fr_batch *b, uint32_t itemsPerWave);
+
 /* records an event on the batch stream after the last enqueued work; query / wait on it */
 int umpa_fr_batch_record(umpa_fr_batch *b, uint64_t *ticket);
 int umpa_fr_batch_query(umpa_fr_batch *b, uint64_t ticket, int *done);
@@ -242,8 +248,9 @@ typedef struct umpa_fr_sched_params {
   int32_t priority;  /* heavy bricks at raised wave priority */
   int32_t maxParts;  /* split heavy bricks into at most this many slice ranges (fast math) */
   float splitAt;     /* ... above splitAt x one wavefront's fair share of the launch */
-  /* Kernel knobs; 0 selects the default in each.  Batches use their own kernel with the table
-   * in LDS and honour boxFixed only. */
+  /* Kernel knobs; 0 selects the default in each.  Batches keep the table in LDS and run the
+   * per-pair kernel, or the carried queues when `kernel` names them (fast math) and every
+   * job's slices fit one 2 GiB buffer resource. */
   int32_t kernel;      /* UMPA_FR_KERNEL_*: insertion kernel; one that does not cover the
                           variant (mode, strategy, footprint) runs as UMPA_FR_KERNEL_PAIR */
   int32_t tableGlobal; /* 1: blob table read from global memory instead of LDS */

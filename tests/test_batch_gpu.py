@@ -55,6 +55,64 @@ def test_exact_batch_matches_oracle(chunk, graph):
         assert gw.tobytes() == rw.tobytes(), (box, chunk, graph)
 
 
+@pytest.mark.parametrize("items,graph", [(1, False), (3, True), (64, False)])
+def test_bounded_launches_match_oracle(items, graph):
+    """Bounded launches (every wavefront leaves after `items` work items, the grid covers the
+    items): the same bits as the persistent launch, with or without the HIP graph."""
+    s = T.settings()
+    host, jobs = make_jobs(SPECS, s, seed=9)
+    b = F.FRBatch(0)
+    b.set_wave_items(items)
+    b.plan(jobs, 11, F.FR_EXACT, graph=graph)
+    b.execute()
+    b.synchronize()
+    for (box, fft, spaces, desc, table, _), j in zip(host, jobs):
+        rv, rw, _ = T.run_oracle(s, fft, spaces, table, box)
+        assert j.volume.cpu().numpy().tobytes() == rv.tobytes(), (box, items)
+        assert j.weights.cpu().numpy().tobytes() == rw.tobytes(), (box, items)
+
+
+def test_two_batches_on_their_streams_overlap_without_changing_results():
+    """Two batches (two streams) with bounded launches enqueued back to back, as the server's
+    two lanes run: each job's result is the oracle's, bit for bit."""
+    s = T.settings()
+    host_a, jobs_a = make_jobs(SPECS[:2], s, seed=13)
+    host_b, jobs_b = make_jobs(SPECS[2:], s, seed=17)
+    ba, bb = F.FRBatch(0), F.FRBatch(0)
+    for b in (ba, bb):
+        b.set_wave_items(2)
+    for c in range(0, 80, 9):
+        for b, jobs in ((ba, jobs_a), (bb, jobs_b)):
+            b.step(jobs, [min(c, j.n_spaces) for j in jobs],
+                   [min(c + 9, j.n_spaces) for j in jobs], F.FR_EXACT)
+    ba.synchronize()
+    bb.synchronize()
+    for host, jobs in ((host_a, jobs_a), (host_b, jobs_b)):
+        for (box, fft, spaces, desc, table, _), j in zip(host, jobs):
+            rv, rw, _ = T.run_oracle(s, fft, spaces, table, box)
+            assert j.volume.cpu().numpy().tobytes() == rv.tobytes(), box
+            assert j.weights.cpu().numpy().tobytes() == rw.tobytes(), box
+
+
+@pytest.mark.parametrize("half,fixed", [(False, 0), (True, 0), (False, 1)])
+def test_batch_carried_queues_within_tolerance(half, fixed):
+    """Batches on the carried-queue kernel (the knob the batch tuner tries for fast math), FP32
+    and FP16 slices, sliding or fixed boxes, split heavy bricks: within 1e-5 of the oracle (on
+    the dequantized slices for FP16)."""
+    s = T.settings()
+    host, jobs = make_jobs(SPECS, s, half=half, seed=21)
+    b = F.FRBatch(0)
+    b.set_sched_params(F.FR_FAST_MATH,
+                       F.SchedParams(0.5, 1, 16, 0.25, F.FR_KERNEL_CARRY, 0, fixed))
+    b.plan(jobs, 13, F.FR_FAST_MATH, graph=True)
+    b.execute()
+    b.synchronize()
+    for (box, fft, spaces, desc, table, _), j in zip(host, jobs):
+        rv, rw, _ = T.run_oracle(s, fft, spaces, table, box)
+        assert T.rel_l2(j.volume.cpu().numpy(), rv) <= 1e-5, box
+        assert T.rel_l2(j.weights.cpu().numpy(), rw) <= 1e-5, box
+
+
 def test_graph_replays_accumulate():
     """Each execute replays the whole chunk loop onto the volumes: two replays = the job's
     spaces inserted twice, still in order."""

@@ -49,12 +49,17 @@ def solo(job_host, strategy, chunk=64):
     return job.volume.cpu().numpy(), job.weights.cpu().numpy()
 
 
-@pytest.mark.parametrize("policy", ["srpt", "aged", "oldest", "fair"])
-def test_exact_alone_equals_mix_and_oracle(policy):
+@pytest.mark.parametrize("policy,lanes", [("srpt", 2), ("aged", 2), ("oldest", 2), ("fair", 2),
+                                          ("aged", 1)])
+def test_exact_alone_equals_mix_and_oracle(policy, lanes):
+    """Jobs served together (the smallest on the fast lane unless one lane, the others on the
+    main lane, chunks copied into each job's ring) give the bits each gives served alone, and
+    FRCPU's."""
     chunk = 64
     mix = make_jobs(chunk=chunk)
     srv = S.Server(chunk=chunk, strategy=F.FR_EXACT, depth=3, policy=policy,
-                   tick_work=2 * 64 * 33 ** 2)
+                   tick_work=2 * 64 * 33 ** 2, fast_seconds=2e-2 if lanes == 2 else None,
+                   ring_bytes=2 * chunk * 17 * 32 * 8)   # rings of 1-2 slots, reused
     done = srv.run([j for j, _ in mix], realtime=True)
     assert len(done) == len(SPECS)
     alone = make_jobs(chunk=chunk)

@@ -147,6 +147,7 @@ struct umpa_fr_batch {
   std::map<std::tuple<int, int, int, float>, umpa::host::BrickSet> cands;
   std::vector<umpa::host::BrickOrigin> schedBuf;  // make_schedule's output, reused
   umpa::host::SchedParams params[2];  // schedule knobs for exact / fast-math launches
+  int waveItems = 0;  // items per wave before it leaves (0: persistent workgroups)
 
   // static plan
   std::vector<std::vector<LaunchImage>> steps;
@@ -220,6 +221,14 @@ void build_step(umpa_fr_batch *b, const std::vector<JobInfo> &infos,
     const auto &items = b->schedBuf;
     LaunchImage li{};
     li.v = first.v;
+    // the carried-queue kernel reads a job's slices through one buffer resource: it runs only
+    // when the knob names it and every job's slices fit one
+    bool carryFits = true;
+    for (const umpa::FrJob &rec : recs)
+      carryFits = carryFits && (uint64_t)rec.nProj * (uint64_t)rec.sliceElems *
+                                   (li.v.half ? 4u : 8u) <= 0x7fff0000ull;
+    li.v.kernel = sp.kernel == UMPA_FR_KERNEL_CARRY && carryFits ? UMPA_FR_KERNEL_CARRY
+                                                                 : UMPA_FR_KERNEL_PAIR;
     li.table = first.table;
     li.nTable = first.nTable;
     li.nItems = (int)items.size();
@@ -254,6 +263,7 @@ hipError_t enqueue(umpa_fr_batch *b, const LaunchImage &li, char *base, bool cou
   l.updates = b->dUpdates;
   l.table = li.table;
   l.nTable = li.nTable;
+  l.waveItems = b->waveItems;
   umpa::FrVariant v = li.v;
   v.count = count;
   if (!capturing) {
@@ -560,8 +570,8 @@ static int batch_autotune_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32
   if (e == hipSuccess) e = hipEventCreate(&t0);
   if (e == hipSuccess) e = hipEventCreate(&t1);
   double bestMs = 1e300;
-  // batches run their own kernel with the table in LDS: of the kernel knobs only boxFixed
-  // applies, tried (fast math) over the fastest schedule
+  // batches keep the table in LDS and run the per-pair kernel or (fast math) the carried
+  // queues: those and boxFixed are tried over the fastest schedule
   std::vector<umpa::host::SchedParams> cands;
   for (auto c : umpa::host::tuning_candidates(fastMath)) {
     c.kernel = UMPA_FR_KERNEL_AUTO;
@@ -572,14 +582,17 @@ static int batch_autotune_impl(umpa_fr_batch *b, const umpa_fr_job *jobs, uint32
     });
     if (!dup) cands.push_back(c);
   }
-  const size_t nSched = cands.size(), nCand = nSched + (fastMath ? 1 : 0);
+  const size_t nSched = cands.size(), nCand = nSched + (fastMath ? 3 : 0);
   try {
     for (size_t ci = 0; ci < nCand; ++ci) {
       if (e != hipSuccess) break;
-      if (ci == nSched) {  // the fastest schedule with fixed boxes
-        umpa::host::SchedParams c = best;
-        c.boxFixed = true;
-        cands.push_back(c);
+      if (ci == nSched) {  // the fastest schedule: fixed boxes, carried queues, both
+        for (int k = 1; k <= 3; ++k) {
+          umpa::host::SchedParams c = best;
+          c.boxFixed = (k & 1) != 0;
+          c.kernel = (k & 2) ? UMPA_FR_KERNEL_CARRY : UMPA_FR_KERNEL_AUTO;
+          cands.push_back(c);
+        }
       }
       const umpa::host::SchedParams cand = cands[ci];
       std::vector<char> blob;
@@ -641,6 +654,13 @@ int umpa_fr_batch_set_sched_params(umpa_fr_batch *b, int strategy, const umpa_fr
       !umpa::host::sched_params_valid(*p))
     return UMPA_ERR_INVALID_ARG;
   b->params[strategy == UMPA_FR_FAST_MATH ? 1 : 0] = umpa::host::from_c(*p);
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_set_wave_items(umpa_fr_batch *b, uint32_t itemsPerWave)
+{
+  if (!b || itemsPerWave > (1u << 20)) return UMPA_ERR_INVALID_ARG;
+  b->waveItems = (int)itemsPerWave;
   return UMPA_OK;
 }
 

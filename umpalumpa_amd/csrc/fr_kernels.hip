@@ -724,7 +724,7 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
   WaveScratch &ws = s_wave[threadIdx.x >> 6];
   constexpr int kPix = kHalf ? 4 : 8;
 
-  for (;;) {
+  for (int taken = 0; l.waveItems == 0 || taken < l.waveItems; ++taken) {
     int b = 0;
     if (lane == 0) b = atomicAdd(l.counter, 1);
     b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
@@ -1000,7 +1000,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void fr_insert_carry_kernel(FrJob j
   CarryScratch &ws = s_wave[wv];
   CarrySlot *slots = s_slot[wv];
 
-  for (;;) {
+  for (int taken = 0; l.waveItems == 0 || taken < l.waveItems; ++taken) {
     int b = 0;
     if (lane == 0) b = atomicAdd(l.counter, 1);
     b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
@@ -1186,7 +1186,7 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
   const int cxi = c.cMaxVolumeIndexX / 2, cyi = c.cMaxVolumeIndexYZ / 2;
   const float r2 = c.cBlobRadius * c.cBlobRadius;
 
-  for (;;) {
+  for (int taken = 0; l.waveItems == 0 || taken < l.waveItems; ++taken) {
     int b = 0;
     if (lane == 0) b = atomicAdd(l.counter, 1);
     b = __builtin_amdgcn_readfirstlane(__shfl(b, 0));
@@ -1360,20 +1360,19 @@ static bool staged_fits(const FrJob &job)
 // launch also needs the chunk's slices under 2 GiB, one buffer resource).
 static const void *pick_carry(const FrVariant &v, int nTable)
 {
-  if (v.mode != kModeLookup || !v.fastMath || v.batched) return nullptr;
+  if (v.mode != kModeLookup || !v.fastMath) return nullptr;
   if (!table_in_lds(v, nTable) || nTable > kCarryMaxTable) return nullptr;
-  if (v.half) {
-    if (v.nb == 3) return v.count ? (const void *)fr_insert_carry_kernel<3, true, true, false>
-                                  : (const void *)fr_insert_carry_kernel<3, true, false, false>;
-    if (v.nb == 4) return v.count ? (const void *)fr_insert_carry_kernel<4, true, true, false>
-                                  : (const void *)fr_insert_carry_kernel<4, true, false, false>;
-    return nullptr;
+  if (v.nb != 3 && v.nb != 4) return nullptr;
+#define UMPA_CARRY(NB, H, B)                                                                 \
+  (v.count ? (const void *)fr_insert_carry_kernel<NB, H, true, B>                            \
+           : (const void *)fr_insert_carry_kernel<NB, H, false, B>)
+  if (v.batched) {
+    if (v.half) return v.nb == 3 ? UMPA_CARRY(3, true, true) : UMPA_CARRY(4, true, true);
+    return v.nb == 3 ? UMPA_CARRY(3, false, true) : UMPA_CARRY(4, false, true);
   }
-  if (v.nb == 3) return v.count ? (const void *)fr_insert_carry_kernel<3, false, true, false>
-                                : (const void *)fr_insert_carry_kernel<3, false, false, false>;
-  if (v.nb == 4) return v.count ? (const void *)fr_insert_carry_kernel<4, false, true, false>
-                                : (const void *)fr_insert_carry_kernel<4, false, false, false>;
-  return nullptr;
+  if (v.half) return v.nb == 3 ? UMPA_CARRY(3, true, false) : UMPA_CARRY(4, true, false);
+  return v.nb == 3 ? UMPA_CARRY(3, false, false) : UMPA_CARRY(4, false, false);
+#undef UMPA_CARRY
 }
 
 static bool carry_fits(const FrJob &job, bool half)
@@ -1407,8 +1406,10 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
   int waves = fr_waves(v.mode, v.half, v.fastMath);
   // kKernelAuto: the staged kernel for the exact strategy, the carried queues for fast-math
   // FP32 slices (FP16 slices run faster on the per-pair kernel: 1.22 vs 1.26 ms per 1,000-slice
-  // chunk at box 256)
+  // chunk at box 256); batches run the per-pair kernel unless they name the carried queues
+  // (their host checks that every job's slices fit one buffer resource)
   const int want = v.kernel != kKernelAuto ? v.kernel
+                   : v.batched            ? kKernelPair
                    : !v.fastMath          ? kKernelStaged
                    : v.half               ? kKernelPair
                                           : kKernelCarry;
@@ -1420,8 +1421,17 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
   } else if (carry && carry_fits(job, v.half)) {
     k = carry;
   }
-  int blocks = (l.nItems + waves - 1) / waves;
-  if (blocks > numCUs) blocks = numCUs;
+  // persistent workgroups, one per CU, until the items run out; with l.waveItems, every wave
+  // leaves after that many items and the grid covers the items (CUs come free for other
+  // streams' launches as workgroups end)
+  int blocks;
+  if (l.waveItems > 0) {
+    const int64_t per = (int64_t)waves * l.waveItems;
+    blocks = (int)((l.nItems + per - 1) / per);
+  } else {
+    blocks = (l.nItems + waves - 1) / waves;
+    if (blocks > numCUs) blocks = numCUs;
+  }
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(fr_reset_counter, dim3(1), dim3(1), 0, st, l.counter);
   FrJob j = job;

@@ -41,6 +41,7 @@ struct FrLaunch {
   unsigned long long *updates;  // voxel-update counter (counting variant only)
   const float *table;   // blob table (kLookup)
   int nTable;
+  int waveItems;        // items per wave before it leaves (0: until the items run out)
 };
 
 // Largest blob table kept in LDS (floats); batches require it there.

@@ -386,6 +386,13 @@ class FRBatch:
         check(_lib.load().umpa_fr_batch_set_sched_params(self._h, strategy, C.byref(p)),
               "set_sched_params")
 
+    def set_wave_items(self, items_per_wave: int) -> None:
+        """Bounded launches (umpa_fr_batch_set_wave_items): every wavefront leaves after that
+        many work items (0: persistent workgroups), so another stream's launches get CUs while
+        this batch's launches run."""
+        check(_lib.load().umpa_fr_batch_set_wave_items(self._h, items_per_wave),
+              "set_wave_items")
+
     def num_steps(self) -> int:
         v = C.c_uint32(0)
         check(_lib.load().umpa_fr_batch_num_steps(self._h, C.byref(v)), "num_steps")
