@@ -36,11 +36,11 @@ operation (fast-math and exact strategies, same chunk size), and p50/p99 job lat
 1,000-50,000 particles log-uniform, blob radius {1.5, 1.9, 2.5}, symmetry {C1, C4, D2}, FP16
 slices streamed in chunks of 512, HBM budget 200 GB) through the continuous-batching server:
 200 jobs (--jobs); the arrival rate is swept relative to the measured capacity (all jobs
-present at once), with three Poisson arrival sequences per rate and four tick policies
-(shortest remaining work first, the default; the same with ageing; oldest first; every job
-its ring window per tick); the line reports the capacity and, per policy and rate, p50/p99
-job latency and the p50/p99 of per-job slowdown (latency over the job's runtime served alone;
-median and worst over the sequences).
+present at once; --rates), with three Poisson arrival sequences per rate and the tick policies
+named by --policies (srpt: shortest remaining work first, the default; aged: the same with
+ageing; oldest: oldest first; fair: every job its ring window per tick); the line reports the
+capacity and, per policy and rate, p50/p99 job latency and the p50/p99 of per-job slowdown
+(latency over the job's runtime served alone; median and worst over the sequences).
 
 --workload pipeline: the harness's chunk loop from particle images (examples/fourier_reconstruction/
 fr.cpp:46-62): 1,000 float images 256 x 256 per step -> R2C FFT and crop to 128 x 256 with
@@ -84,8 +84,10 @@ def parse():
                         "workload's bar); exact: bit-identical to FRCPU; f16: FP16 slices")
     p.add_argument("--workload", default="plain",
                    choices=["plain", "batched", "serving", "large_d7", "pipeline"])
-    p.add_argument("--rates", default="0.5,0.8,0.9,1.0",
+    p.add_argument("--rates", default="0.5,0.8,1.0",
                    help="serving: arrival rates as fractions of the measured capacity")
+    p.add_argument("--policies", default="srpt,aged",
+                   help="serving: tick policies (srpt, aged, oldest, fair)")
     p.add_argument("--hbm-budget", type=float, default=200e9)
     p.add_argument("--jobs", type=int, default=16)
     p.add_argument("--retune", action="store_true", help="ignore the persisted tuning cache")
@@ -571,7 +573,7 @@ def run_serving(args):
     from umpalumpa_amd import serve as S
     n_jobs = args.jobs if args.jobs != 16 else 200
     summ = serving_summary(torch, S, F, n_jobs, [float(x) for x in args.rates.split(",")],
-                           ("srpt", "aged", "oldest", "fair"), SERVING_SEEDS, args.hbm_budget)
+                           tuple(args.policies.split(",")), SERVING_SEEDS, args.hbm_budget)
     result = {
         "metric": "GVoxel-updates/s + HBM GB/s (box 256); p50/p99 job latency, batched vs plain",
         "value": summ["capacity_jobs_per_s"],
