@@ -44,6 +44,8 @@ class Job:
     arrival: float = 0.0           # seconds after the server's start
     # filled by the server
     admitted: float | None = None
+    ready: float | None = None     # admission finished (spaces resident)
+    started: float | None = None   # first step enqueued
     done: float | None = None
     volume: object = None
     weights: object = None
@@ -256,6 +258,7 @@ class Server:
         job._chunk_spaces = chunks
         job._spaces = dev
         job._prep = None
+        job.ready = time.perf_counter() - self._t0
         return True
 
     def _window_end(self, job: Job) -> int:
@@ -324,6 +327,8 @@ class Server:
         ticket = lane.batch.record()
         lane.inflight.append((ticket, work))
         for j, _, e in plan:
+            if j.started is None:
+                j.started = time.perf_counter() - self._t0
             j._last_ticket = ticket
             j._cursor = e
         return work
@@ -338,7 +343,7 @@ class Server:
         busy_since, done_work = None, 0.0   # main lane: the insertion rate
         main = self._main
         n_main = 0                          # admitted main-lane jobs
-        t0 = time.perf_counter()
+        t0 = self._t0 = time.perf_counter()
         for ln in lanes:
             ln.batch.set_count_updates(count_updates)
         self.total_updates = 0
