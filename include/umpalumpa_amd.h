@@ -254,8 +254,9 @@ typedef struct umpa_fr_sched_params {
   int32_t kernel;      /* UMPA_FR_KERNEL_*: insertion kernel; one that does not cover the
                           variant (mode, strategy, footprint) runs as UMPA_FR_KERNEL_PAIR */
   int32_t tableGlobal; /* 1: blob table read from global memory instead of LDS */
-  int32_t boxFixed;    /* 1: fast-math candidate boxes stay at the blob's bounding box instead
-                          of sliding inside the slice (no masking of out-of-box candidates) */
+  int32_t boxFixed;    /* 1: candidate boxes stay at the blob's bounding box instead of
+                          sliding inside the slice (fast-math kernels and the staged exact
+                          kernel; sliding needs no masking of out-of-box candidates) */
 } umpa_fr_sched_params;
 
 enum {

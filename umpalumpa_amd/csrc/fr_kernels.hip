@@ -291,10 +291,12 @@ __device__ __forceinline__ float candidate_weight(float d2, float r2, int nTable
 // skipped pixels past the buffer end, as zeros).  Fast-math strategy (finite slice values):
 // the table index is clamped into the zero entry, so out-of-blob candidates weigh 0 without a
 // test.  Rows are read whole; pixels past the slice end read 0 through the bounds-checked
-// buffer.
+// buffer.  The box slides inside the slice (see below) when kShiftAlways (the caller's
+// instantiation for jobs with FrJob::shiftBox) or when kShiftable and the job's shiftBox is set
+// (a run-time test per call).
 // Returns false when the footprint is wider than NB.
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds,
-          bool kShiftable = true>
+          bool kShiftable = true, bool kShiftAlways = false>
 __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
                                            __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
                                            float qx, float qy, float zSqr, float dw,
@@ -310,7 +312,7 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
   constexpr int kPix = kHalf ? 4 : 8;
   bool fits = true;
   float xd2[kNB], yz[kNB];
-  if (kShiftable && a.shiftBox) {
+  if (kShiftAlways || (kShiftable && a.shiftBox)) {
     // The box slides inside the slice (slices of at least kNB x kNB pixels): it covers the
     // reference's bounding box [minX, maxX] x [minY, maxY] whenever that has at most kNB
     // columns and rows (the host sets shiftBox only where floor(2r) + 1 <= kNB), and every
@@ -1132,7 +1134,7 @@ struct WaveStage {
   uint32_t key[kStage];  // voxel index in the brick (bits 0-15), space ordinal (bits 16-31)
 };
 
-template <int kNB>
+template <int kNB, bool kShift>
 __device__ __forceinline__ void staged_gather(const FrJob &a, int nTable,
                                               __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
                                               WaveScratch &ws, const WaveStage &st, int lane,
@@ -1149,10 +1151,10 @@ __device__ __forceinline__ void staged_gather(const FrJob &a, int nTable,
     // reference's, and wb * 1 = wb, so the choice per batch changes no bit
     bool fits;
     if (__all(dw == 1.f)) {
-      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true, false>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true, false, kShift>(a, nTable, rsrc, tbl, qx,
                                                                        qy, zs, dw, o, off);
     } else {
-      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true, false>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true, false, kShift>(a, nTable, rsrc, tbl, qx,
                                                                         qy, zs, dw, o, off);
     }
     if (!fits)
@@ -1175,7 +1177,11 @@ __device__ __forceinline__ void staged_gather(const FrJob &a, int nTable,
   }
 }
 
-template <int kNB>
+// kShift: the job's candidate boxes slide inside the slice (FrJob::shiftBox), a separate
+// instantiation: decided per candidate at run time, the choice made the compiler reschedule the
+// kernel (116 -> 83 VGPRs, 1.49 -> 1.92 ms per 1,000-slice chunk at box 256); as its own
+// instantiation the sliding box takes the chunk from 1.502 to 1.445 ms, bit for bit the same.
+template <int kNB, bool kShift>
 __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJob a, FrLaunch l)
 {
   constexpr int kWaves = kStagedWaves;
@@ -1259,7 +1265,7 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
           waiting += __popcll(hm);
           if (waiting >= 64) {
             wave_sync();
-            staged_gather<kNB>(a, nTable, rsrc, tbl, ws, st, lane, head, 64);
+            staged_gather<kNB, kShift>(a, nTable, rsrc, tbl, ws, st, lane, head, 64);
             wave_sync();
             head = (head + 64) & (kStage - 1);
             waiting -= 64;
@@ -1271,7 +1277,7 @@ __global__ __launch_bounds__(64 * kStagedWaves) void fr_insert_staged_kernel(FrJ
     }
     if (waiting > 0) {
       wave_sync();
-      staged_gather<kNB>(a, nTable, rsrc, tbl, ws, st, lane, head, waiting);
+      staged_gather<kNB, kShift>(a, nTable, rsrc, tbl, ws, st, lane, head, waiting);
       wave_sync();
     }
     store_brick(ws, a, bk, lane, false);
@@ -1357,12 +1363,14 @@ static const void *pick_kernel(const FrVariant &v, int nTable)
 
 // The exact FP32 lookup kernel with cross-space gather batches, for the variants it covers
 // (the launch also needs the chunk's slices under 2 GiB, one buffer resource).
-static const void *pick_staged(const FrVariant &v, int nTable)
+static const void *pick_staged(const FrVariant &v, int nTable, bool shift)
 {
   if (v.mode != kModeLookup || v.half || v.fastMath || v.count || v.batched) return nullptr;
   if (!table_in_lds(v, nTable) || nTable > kStagedMaxTable) return nullptr;
-  if (v.nb == 3) return (const void *)fr_insert_staged_kernel<3>;
-  if (v.nb == 4) return (const void *)fr_insert_staged_kernel<4>;
+  if (v.nb == 3) return shift ? (const void *)fr_insert_staged_kernel<3, true>
+                              : (const void *)fr_insert_staged_kernel<3, false>;
+  if (v.nb == 4) return shift ? (const void *)fr_insert_staged_kernel<4, true>
+                              : (const void *)fr_insert_staged_kernel<4, false>;
   return nullptr;
 }
 
@@ -1402,9 +1410,11 @@ hipError_t prepare_fr_insert(const FrVariant &v, int nTable)
   if (!table_in_lds(v, nTable)) return hipSuccess;
   const int bytes = (int)((nTable + 1) * sizeof(float));
   hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  const void *staged = pick_staged(v, nTable);
-  if (e == hipSuccess && staged)
-    e = hipFuncSetAttribute(staged, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  for (bool shift : { false, true }) {
+    const void *staged = pick_staged(v, nTable, shift);
+    if (e == hipSuccess && staged)
+      e = hipFuncSetAttribute(staged, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  }
   const void *carry = pick_carry(v, nTable);
   if (e == hipSuccess && carry)
     e = hipFuncSetAttribute(carry, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
@@ -1428,7 +1438,7 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
                    : !v.fastMath          ? kKernelStaged
                    : v.half               ? kKernelPair
                                           : kKernelCarry;
-  const void *staged = want == kKernelStaged ? pick_staged(v, l.nTable) : nullptr;
+  const void *staged = want == kKernelStaged ? pick_staged(v, l.nTable, job.shiftBox != 0) : nullptr;
   const void *carry = want == kKernelCarry ? pick_carry(v, l.nTable) : nullptr;
   if (staged && staged_fits(job)) {
     k = staged;
