@@ -229,6 +229,9 @@ void build_step(umpa_fr_batch *b, const std::vector<JobInfo> &infos,
                                    (li.v.half ? 4u : 8u) <= 0x7fff0000ull;
     li.v.kernel = sp.kernel == UMPA_FR_KERNEL_CARRY && carryFits ? UMPA_FR_KERNEL_CARRY
                                                                  : UMPA_FR_KERNEL_PAIR;
+    // the sliding-box instantiation only when every job of the launch slides
+    li.v.shift = std::all_of(recs.begin(), recs.end(),
+                             [](const umpa::FrJob &r) { return r.shiftBox != 0; });
     li.table = first.table;
     li.nTable = first.nTable;
     li.nItems = (int)items.size();

@@ -291,12 +291,11 @@ __device__ __forceinline__ float candidate_weight(float d2, float r2, int nTable
 // skipped pixels past the buffer end, as zeros).  Fast-math strategy (finite slice values):
 // the table index is clamped into the zero entry, so out-of-blob candidates weigh 0 without a
 // test.  Rows are read whole; pixels past the slice end read 0 through the bounds-checked
-// buffer.  The box slides inside the slice (see below) when kShiftAlways (the caller's
-// instantiation for jobs with FrJob::shiftBox) or when kShiftable and the job's shiftBox is set
-// (a run-time test per call).
+// buffer.  With kShift (the caller's instantiation for jobs with FrJob::shiftBox) the box slides
+// inside the slice (see below).
 // Returns false when the footprint is wider than NB.
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kUnitWeight, bool kTableLds,
-          bool kShiftable = true, bool kShiftAlways = false>
+          bool kShift>
 __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
                                            __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
                                            float qx, float qy, float zSqr, float dw,
@@ -312,7 +311,7 @@ __device__ __forceinline__ bool gather_box(const FrJob &a, int nTable,
   constexpr int kPix = kHalf ? 4 : 8;
   bool fits = true;
   float xd2[kNB], yz[kNB];
-  if (kShiftAlways || (kShiftable && a.shiftBox)) {
+  if constexpr (kShift) {
     // The box slides inside the slice (slices of at least kNB x kNB pixels): it covers the
     // reference's bounding box [minX, maxX] x [minY, maxY] whenever that has at most kNB
     // columns and rows (the host sets shiftBox only where floor(2r) + 1 <= kNB), and every
@@ -707,7 +706,7 @@ __device__ __forceinline__ int slab_queue(WaveScratch &ws, const FrJob &a, const
 }
 
 template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount, bool kTableLds,
-          bool kBatched>
+          bool kBatched, bool kShift>
 __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_insert_kernel(FrJob job0, FrLaunch l)
 {
   constexpr int kWaves = fr_waves(kMode, kHalf, kFastMath);
@@ -844,10 +843,10 @@ __global__ __launch_bounds__(64 * fr_waves(kMode, kHalf, kFastMath)) void fr_ins
             } else {
               bool fits;
               if (unit) {
-                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds, kFastMath>(
+                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, true, kTableLds, kShift>(
                   a, nTable, rsrc, tbl, qx, qy, zs, sp.weight, o);
               } else {
-                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds, kFastMath>(
+                fits = gather_box<kMode, kOrder, kNB, kHalf, kFastMath, false, kTableLds, kShift>(
                   a, nTable, rsrc, tbl, qx, qy, zs, sp.weight, o);
               }
               if (!fits)
@@ -934,7 +933,7 @@ __device__ __forceinline__ int slab_queue_carry(CarryScratch &ws, const FrJob &a
 
 // One batch of `n` (1..64) queue entries from `start`: the rest of the predicate, the gather and
 // the accumulation, space by space.
-template <int kNB, bool kHalf, bool kCount>
+template <int kNB, bool kHalf, bool kCount, bool kShift>
 __device__ __forceinline__ void carry_batch(CarryScratch &ws, const CarrySlot *slots,
                                             const FrJob &a, const int4 &bo, int bz, int nTable,
                                             __amdgpu_buffer_rsrc_t rsrc, const float *tbl,
@@ -960,10 +959,10 @@ __device__ __forceinline__ void carry_batch(CarryScratch &ws, const CarrySlot *s
     const float dw = sp.weight;
     bool fits;
     if (__all(!hit || dw == 1.f)) {
-      fits = gather_box<kModeLookup, 0, kNB, kHalf, true, true, true>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, kHalf, true, true, true, kShift>(a, nTable, rsrc, tbl, qx,
                                                                        qy, zs, dw, o, sliceOff);
     } else {
-      fits = gather_box<kModeLookup, 0, kNB, kHalf, true, false, true>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, kHalf, true, false, true, kShift>(a, nTable, rsrc, tbl, qx,
                                                                         qy, zs, dw, o, sliceOff);
     }
     if (!fits)
@@ -992,7 +991,7 @@ __device__ __forceinline__ void carry_batch(CarryScratch &ws, const CarrySlot *s
   if constexpr (kCount) nupd += (unsigned long long)__popcll(__ballot(hit));
 }
 
-template <int kNB, bool kHalf, bool kCount, bool kBatched>
+template <int kNB, bool kHalf, bool kCount, bool kBatched, bool kShift>
 __global__ __launch_bounds__(64 * kMaxWaves) void fr_insert_carry_kernel(FrJob job0, FrLaunch l)
 {
   constexpr int kWaves = kMaxWaves;
@@ -1061,7 +1060,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void fr_insert_carry_kernel(FrJob j
         if (pendSpaces == 2) {
           // the slot about to be rewritten still holds a pending space: gather everything
           wave_sync();
-          carry_batch<kNB, kHalf, kCount>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane, 0,
+          carry_batch<kNB, kHalf, kCount, kShift>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane, 0,
                                           pending, nupd);
           pending = 0;
           pendSpaces = 0;
@@ -1086,7 +1085,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void fr_insert_carry_kernel(FrJob j
         const int full = pending >> 6;
         if (full) {
           for (int t = 0; t < full; ++t)
-            carry_batch<kNB, kHalf, kCount>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane,
+            carry_batch<kNB, kHalf, kCount, kShift>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane,
                                             t * 64, 64, nupd);
           // the leftovers (< 64, all of the space just queued) move to the queue's front
           const int left = pending - (full << 6);
@@ -1100,7 +1099,7 @@ __global__ __launch_bounds__(64 * kMaxWaves) void fr_insert_carry_kernel(FrJob j
       }
     }
     if (pending) {
-      carry_batch<kNB, kHalf, kCount>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane, 0, pending,
+      carry_batch<kNB, kHalf, kCount, kShift>(ws, slots, a, bo, bz, nTable, rsrc, tbl, lane, 0, pending,
                                       nupd);
       wave_sync();
     }
@@ -1151,10 +1150,10 @@ __device__ __forceinline__ void staged_gather(const FrJob &a, int nTable,
     // reference's, and wb * 1 = wb, so the choice per batch changes no bit
     bool fits;
     if (__all(dw == 1.f)) {
-      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true, false, kShift>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, false, false, true, true, kShift>(a, nTable, rsrc, tbl, qx,
                                                                        qy, zs, dw, o, off);
     } else {
-      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true, false, kShift>(a, nTable, rsrc, tbl, qx,
+      fits = gather_box<kModeLookup, 0, kNB, false, false, false, true, kShift>(a, nTable, rsrc, tbl, qx,
                                                                         qy, zs, dw, o, off);
     }
     if (!fits)
@@ -1290,27 +1289,39 @@ __global__ void fr_reset_counter(int *counter) { *counter = 0; }
 // ---- host-side dispatch ------------------------------------------------------------------
 // The kernel is instantiated per (mode, order, footprint width, FP16, fast math, counting, LDS
 // table, batched); `visit` calls `f` with a pointer to the instantiation a variant selects.
-template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount>
+template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount, bool kShift>
 static const void *pick_lds(bool lds, bool batched)
 {
   // batches with a lookup table keep it in LDS (tables up to kMaxLdsTable samples)
   if constexpr (kMode == kModeLookup) {
     if (batched && !lds) return nullptr;
-    if (batched) return (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true, true>;
+    if (batched) return (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true, true, kShift>;
   }
-  if (lds) return (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true, false>;
-  return batched ? (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false, true>
-                 : (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false, false>;
+  if (lds) return (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true, false, kShift>;
+  return batched ? (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false, true, kShift>
+                 : (const void *)fr_insert_kernel<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false, false, kShift>;
+}
+
+// The sliding box (FrJob::shiftBox, every job of the launch) is an instantiation of its own for
+// the fast-math kernels: tested per gather at run time, it cost 2-3 % (fast math 1.229 against
+// 1.251 ms, FP16 1.189 against 1.222 ms per 1,000-slice chunk at box 256).
+template <int kMode, int kOrder, int kNB, bool kHalf, bool kFastMath, bool kCount>
+static const void *pick_shift(bool lds, bool batched, bool shift)
+{
+  if constexpr (kFastMath) {
+    if (shift) return pick_lds<kMode, kOrder, kNB, kHalf, kFastMath, kCount, true>(lds, batched);
+  }
+  return pick_lds<kMode, kOrder, kNB, kHalf, kFastMath, kCount, false>(lds, batched);
 }
 
 template <int kMode, int kOrder, bool kHalf, bool kFastMath, bool kCount>
-static const void *pick_nb(int nb, bool lds, bool batched)
+static const void *pick_nb(int nb, bool lds, bool batched, bool shift)
 {
   switch (nb) {
-  case 3: return pick_lds<kMode, kOrder, 3, kHalf, kFastMath, kCount>(lds, batched);
-  case 4: return pick_lds<kMode, kOrder, 4, kHalf, kFastMath, kCount>(lds, batched);
-  case 5: return pick_lds<kMode, kOrder, 5, kHalf, kFastMath, kCount>(lds, batched);
-  default: return pick_lds<kMode, kOrder, 6, kHalf, kFastMath, kCount>(lds, batched);
+  case 3: return pick_shift<kMode, kOrder, 3, kHalf, kFastMath, kCount>(lds, batched, shift);
+  case 4: return pick_shift<kMode, kOrder, 4, kHalf, kFastMath, kCount>(lds, batched, shift);
+  case 5: return pick_shift<kMode, kOrder, 5, kHalf, kFastMath, kCount>(lds, batched, shift);
+  default: return pick_shift<kMode, kOrder, 6, kHalf, kFastMath, kCount>(lds, batched, shift);
   }
 }
 
@@ -1320,11 +1331,11 @@ template <bool kCount>
 static const void *pick_order(int order, bool batched)
 {
   switch (order) {
-  case 0: return pick_lds<kModePreciseKaiser, 0, 4, false, false, kCount>(false, batched);
-  case 1: return pick_lds<kModePreciseKaiser, 1, 4, false, false, kCount>(false, batched);
-  case 2: return pick_lds<kModePreciseKaiser, 2, 4, false, false, kCount>(false, batched);
-  case 3: return pick_lds<kModePreciseKaiser, 3, 4, false, false, kCount>(false, batched);
-  case 4: return pick_lds<kModePreciseKaiser, 4, 4, false, false, kCount>(false, batched);
+  case 0: return pick_lds<kModePreciseKaiser, 0, 4, false, false, kCount, false>(false, batched);
+  case 1: return pick_lds<kModePreciseKaiser, 1, 4, false, false, kCount, false>(false, batched);
+  case 2: return pick_lds<kModePreciseKaiser, 2, 4, false, false, kCount, false>(false, batched);
+  case 3: return pick_lds<kModePreciseKaiser, 3, 4, false, false, kCount, false>(false, batched);
+  case 4: return pick_lds<kModePreciseKaiser, 4, 4, false, false, kCount, false>(false, batched);
   default: return nullptr;
   }
 }
@@ -1341,19 +1352,19 @@ static const void *pick_kernel(const FrVariant &v, int nTable)
   switch (v.mode) {
   case kModeLookup:
     if (v.count) {
-      if (v.half) return pick_nb<kModeLookup, 0, true, true, true>(v.nb, lds, b);
-      return v.fastMath ? pick_nb<kModeLookup, 0, false, true, true>(v.nb, lds, b)
-                        : pick_nb<kModeLookup, 0, false, false, true>(v.nb, lds, b);
+      if (v.half) return pick_nb<kModeLookup, 0, true, true, true>(v.nb, lds, b, v.shift);
+      return v.fastMath ? pick_nb<kModeLookup, 0, false, true, true>(v.nb, lds, b, v.shift)
+                        : pick_nb<kModeLookup, 0, false, false, true>(v.nb, lds, b, v.shift);
     }
-    if (v.half) return pick_nb<kModeLookup, 0, true, true, false>(v.nb, lds, b);
-    if (v.fastMath) return pick_nb<kModeLookup, 0, false, true, false>(v.nb, lds, b);
-    return pick_nb<kModeLookup, 0, false, false, false>(v.nb, lds, b);
+    if (v.half) return pick_nb<kModeLookup, 0, true, true, false>(v.nb, lds, b, v.shift);
+    if (v.fastMath) return pick_nb<kModeLookup, 0, false, true, false>(v.nb, lds, b, v.shift);
+    return pick_nb<kModeLookup, 0, false, false, false>(v.nb, lds, b, v.shift);
   case kModeFastKaiser:
-    return v.count ? pick_lds<kModeFastKaiser, 0, 4, false, false, true>(false, b)
-                   : pick_lds<kModeFastKaiser, 0, 4, false, false, false>(false, b);
+    return v.count ? pick_lds<kModeFastKaiser, 0, 4, false, false, true, false>(false, b)
+                   : pick_lds<kModeFastKaiser, 0, 4, false, false, false, false>(false, b);
   case kModeNearest:
-    return v.count ? pick_lds<kModeNearest, 0, 4, false, false, true>(false, b)
-                   : pick_lds<kModeNearest, 0, 4, false, false, false>(false, b);
+    return v.count ? pick_lds<kModeNearest, 0, 4, false, false, true, false>(false, b)
+                   : pick_lds<kModeNearest, 0, 4, false, false, false, false>(false, b);
   case kModePreciseKaiser:
     // the counting variant evaluates the same blob order as the plain one
     return v.count ? pick_order<true>(v.order, b) : pick_order<false>(v.order, b);
@@ -1387,8 +1398,10 @@ static const void *pick_carry(const FrVariant &v, int nTable)
   if (!table_in_lds(v, nTable) || nTable > kCarryMaxTable) return nullptr;
   if (v.nb != 3 && v.nb != 4) return nullptr;
 #define UMPA_CARRY(NB, H, B)                                                                 \
-  (v.count ? (const void *)fr_insert_carry_kernel<NB, H, true, B>                            \
-           : (const void *)fr_insert_carry_kernel<NB, H, false, B>)
+  (v.shift ? (v.count ? (const void *)fr_insert_carry_kernel<NB, H, true, B, true>           \
+                      : (const void *)fr_insert_carry_kernel<NB, H, false, B, true>)         \
+           : (v.count ? (const void *)fr_insert_carry_kernel<NB, H, true, B, false>          \
+                      : (const void *)fr_insert_carry_kernel<NB, H, false, B, false>))
   if (v.batched) {
     if (v.half) return v.nb == 3 ? UMPA_CARRY(3, true, true) : UMPA_CARRY(4, true, true);
     return v.nb == 3 ? UMPA_CARRY(3, false, true) : UMPA_CARRY(4, false, true);
@@ -1405,25 +1418,29 @@ static bool carry_fits(const FrJob &job, bool half)
 
 hipError_t prepare_fr_insert(const FrVariant &v, int nTable)
 {
-  const void *k = pick_kernel(v, nTable);
-  if (!k) return hipErrorInvalidValue;
+  if (!pick_kernel(v, nTable)) return hipErrorInvalidValue;
   if (!table_in_lds(v, nTable)) return hipSuccess;
   const int bytes = (int)((nTable + 1) * sizeof(float));
-  hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  hipError_t e = hipSuccess;
+  // every kernel the launch may pick: either box form (a plain launch takes it from its job)
   for (bool shift : { false, true }) {
-    const void *staged = pick_staged(v, nTable, shift);
-    if (e == hipSuccess && staged)
-      e = hipFuncSetAttribute(staged, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    FrVariant w = v;
+    w.shift = shift;
+    for (const void *k : { pick_kernel(w, nTable), pick_staged(w, nTable, shift),
+                           pick_carry(w, nTable) })
+      if (e == hipSuccess && k)
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   }
-  const void *carry = pick_carry(v, nTable);
-  if (e == hipSuccess && carry)
-    e = hipFuncSetAttribute(carry, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   return e;
 }
 
-hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v, int numCUs,
+hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant &v0, int numCUs,
                             hipStream_t st)
 {
+  // a plain launch slides its boxes when its job does; a batch's host sets v.shift when every
+  // job of the launch does
+  FrVariant v = v0;
+  if (!v.batched) v.shift = job.shiftBox != 0;
   const void *k = pick_kernel(v, l.nTable);
   if (!k) return hipErrorInvalidValue;
   const size_t shmem = table_in_lds(v, l.nTable) ? (size_t)(l.nTable + 1) * sizeof(float) : 0;
@@ -1438,7 +1455,7 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
                    : !v.fastMath          ? kKernelStaged
                    : v.half               ? kKernelPair
                                           : kKernelCarry;
-  const void *staged = want == kKernelStaged ? pick_staged(v, l.nTable, job.shiftBox != 0) : nullptr;
+  const void *staged = want == kKernelStaged ? pick_staged(v, l.nTable, v.shift) : nullptr;
   const void *carry = want == kKernelCarry ? pick_carry(v, l.nTable) : nullptr;
   if (staged && staged_fits(job)) {
     k = staged;

@@ -59,6 +59,8 @@ struct FrVariant {
   bool batched;   // jobs from FrLaunch::jobs (else the FrJob argument)
   int kernel;     // kKernel*: which insertion kernel (kKernelAuto: the measured best)
   bool tableGlobal;  // read the blob table from global memory even where LDS holds it
+  bool shift;        // candidate boxes slide inside the slice (FrJob::shiftBox of every job;
+                     // a plain launch takes it from its job)
 };
 
 // Insertion kernels: one queue per (brick, space) pair; queues carried across spaces (fast
