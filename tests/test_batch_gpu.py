@@ -72,6 +72,23 @@ def test_bounded_launches_match_oracle(items, graph):
         assert j.weights.cpu().numpy().tobytes() == rw.tobytes(), (box, items)
 
 
+@pytest.mark.parametrize("workgroups", [1, 7, 200])
+def test_capped_persistent_launches_match_oracle(workgroups):
+    """Persistent launches on at most N workgroups (umpa_fr_batch_set_max_workgroups): the same
+    bits as on every CU."""
+    s = T.settings()
+    host, jobs = make_jobs(SPECS, s, seed=31)
+    b = F.FRBatch(0)
+    b.set_max_workgroups(workgroups)
+    b.plan(jobs, 11, F.FR_EXACT, graph=False)
+    b.execute()
+    b.synchronize()
+    for (box, fft, spaces, desc, table, _), j in zip(host, jobs):
+        rv, rw, _ = T.run_oracle(s, fft, spaces, table, box)
+        assert j.volume.cpu().numpy().tobytes() == rv.tobytes(), (box, workgroups)
+        assert j.weights.cpu().numpy().tobytes() == rw.tobytes(), (box, workgroups)
+
+
 def test_two_batches_on_their_streams_overlap_without_changing_results():
     """Two batches (two streams) with bounded launches enqueued back to back, as the server's
     two lanes run: each job's result is the oracle's, bit for bit."""

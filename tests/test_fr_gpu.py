@@ -246,6 +246,28 @@ def test_symmetry_ops_and_space_order():
     assert _bits_equal(v, ov) and _bits_equal(w, ow)
 
 
+@pytest.mark.parametrize("box,radius", [(64, 1.9), (48, 1.2), (40, 2.5)])
+def test_space_weights(box, radius):
+    """Traverse spaces with weights other than 1 (the record's `weight`, applied per candidate
+    as blob value x weight, fr_cpu_kernels.hpp:143-151), a mix of 1 and random values so that
+    batches hold both: exact bit-identical to the oracle, fast math and FP16 within 1e-5."""
+    s = settings(radius=radius)
+    fft, spaces, desc, table = problem(box, 60, s, seed=int(radius * 100) + box)
+    rng = np.random.default_rng(box)
+    w = rng.uniform(0.25, 2.0, len(spaces)).astype(np.float32)
+    w[rng.random(len(spaces)) < 0.3] = 1.0
+    spaces["weight"] = w
+    ov, ow, _ = run_oracle(s, fft, spaces, table, box)
+    v, wt, _ = run_hip(s, fft, spaces, desc, table, box)
+    assert _bits_equal(v, ov) and _bits_equal(wt, ow)
+    fv, fw, _ = run_hip(s, fft, spaces, desc, table, box, strategy=F.FR_FAST_MATH)
+    assert rel_l2(fv, ov) <= 1e-5 and rel_l2(fw, ow) <= 1e-5
+    h, sc, deq = quantize_f16(fft)
+    hv, hw, _ = run_hip(s, deq, spaces, desc, table, box, strategy=F.FR_FAST_MATH, half=(h, sc))
+    dv, dw_, _ = run_oracle(s, deq, spaces, table, box)
+    assert rel_l2(hv, dv) <= 1e-5 and rel_l2(hw, dw_) <= 1e-5
+
+
 @pytest.mark.parametrize("radius", [0.8, 1.2, 1.5, 2.2, 2.5, 3.2])
 @pytest.mark.parametrize("interp", [F.FR_LOOKUP, F.FR_DYNAMIC])
 def test_radius_sweep(radius, interp):

@@ -142,6 +142,7 @@ PROTOTYPES = [
                                          C.c_int, C.POINTER(SchedParams)]),
     ("umpa_fr_batch_set_sched_params", C.c_int, [_P, C.c_int, C.POINTER(SchedParams)]),
     ("umpa_fr_batch_set_wave_items", C.c_int, [_P, C.c_uint32]),
+    ("umpa_fr_batch_set_max_workgroups", C.c_int, [_P, C.c_uint32]),
     ("umpa_fr_symmetry_operators", C.c_int, [C.c_char_p, _P, C.c_uint32,
                                              C.POINTER(C.c_uint32)]),
     ("umpa_fr_finalize", C.c_int, [_P, _P, _U64, _P, C.c_int, _P]),

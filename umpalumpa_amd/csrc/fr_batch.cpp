@@ -148,6 +148,7 @@ struct umpa_fr_batch {
   std::vector<umpa::host::BrickOrigin> schedBuf;  // make_schedule's output, reused
   umpa::host::SchedParams params[2];  // schedule knobs for exact / fast-math launches
   int waveItems = 0;  // items per wave before it leaves (0: persistent workgroups)
+  int maxWorkgroups = 0;  // persistent launches: at most this many workgroups (0: one per CU)
 
   // static plan
   std::vector<std::vector<LaunchImage>> steps;
@@ -267,6 +268,7 @@ hipError_t enqueue(umpa_fr_batch *b, const LaunchImage &li, char *base, bool cou
   l.table = li.table;
   l.nTable = li.nTable;
   l.waveItems = b->waveItems;
+  l.maxBlocks = b->maxWorkgroups;
   umpa::FrVariant v = li.v;
   v.count = count;
   if (!capturing) {
@@ -664,6 +666,13 @@ int umpa_fr_batch_set_wave_items(umpa_fr_batch *b, uint32_t itemsPerWave)
 {
   if (!b || itemsPerWave > (1u << 20)) return UMPA_ERR_INVALID_ARG;
   b->waveItems = (int)itemsPerWave;
+  return UMPA_OK;
+}
+
+int umpa_fr_batch_set_max_workgroups(umpa_fr_batch *b, uint32_t workgroups)
+{
+  if (!b || workgroups > 65535) return UMPA_ERR_INVALID_ARG;
+  b->maxWorkgroups = (int)workgroups;
   return UMPA_OK;
 }
 

@@ -1473,6 +1473,7 @@ hipError_t launch_fr_insert(const FrJob &job, const FrLaunch &l, const FrVariant
   } else {
     blocks = (l.nItems + waves - 1) / waves;
     if (blocks > numCUs) blocks = numCUs;
+    if (l.maxBlocks > 0 && blocks > l.maxBlocks) blocks = l.maxBlocks;
   }
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(fr_reset_counter, dim3(1), dim3(1), 0, st, l.counter);

@@ -42,6 +42,7 @@ struct FrLaunch {
   const float *table;   // blob table (kLookup)
   int nTable;
   int waveItems;        // items per wave before it leaves (0: until the items run out)
+  int maxBlocks;        // persistent launches: at most this many workgroups (0: one per CU)
 };
 
 // Largest blob table kept in LDS (floats); batches require it there.

@@ -393,6 +393,12 @@ class FRBatch:
         check(_lib.load().umpa_fr_batch_set_wave_items(self._h, items_per_wave),
               "set_wave_items")
 
+    def set_max_workgroups(self, workgroups: int) -> None:
+        """Persistent launches use at most this many workgroups (one per CU; 0: every CU),
+        leaving CUs to another stream (umpa_fr_batch_set_max_workgroups)."""
+        check(_lib.load().umpa_fr_batch_set_max_workgroups(self._h, workgroups),
+              "set_max_workgroups")
+
     def num_steps(self) -> int:
         v = C.c_uint32(0)
         check(_lib.load().umpa_fr_batch_num_steps(self._h, C.byref(v)), "num_steps")
