@@ -9,8 +9,11 @@ batch step (``umpa_fr_batch_step``): one launch per group of jobs sharing settin
 that arrive while others run are inserted by the same launches.  A job's voxels receive its
 spaces in index order whatever else is in flight, so its result is the same alone and in the
 mix (bit for bit under the exact strategy).  Ticks are enqueued ahead of the GPU (up to
-``depth`` in flight) and completions are noticed through stream tickets, so the host loop
-never waits on an idle GPU.  Small jobs (under about 20 ms of work) run on a second lane, a
+``depth`` in flight) and completions are noticed through stream events, so the host loop
+never waits on an idle GPU.  The loop only plans ticks: each lane's stream work (ring copies,
+the step's work-item schedule, the launches) runs on a thread of its own, so a step over large
+volumes (hundreds of thousands of work items, milliseconds of host work) does not hold up
+admissions or the other lane.  Small jobs (under about 20 ms of work) run on a second lane, a
 batch with its own stream, whose launches take CUs as the main lane's bounded launches free
 them, instead of queueing behind its ticks.  By default each tick serves the jobs with the
 least remaining work first ("srpt"), so short jobs overtake long ones; "aged" lets a long job
@@ -45,7 +48,7 @@ class Job:
     # filled by the server
     admitted: float | None = None
     ready: float | None = None     # admission finished (spaces resident)
-    started: float | None = None   # first step enqueued
+    started: float | None = None   # first tick submitted
     done: float | None = None
     volume: object = None
     weights: object = None
@@ -56,7 +59,7 @@ class Job:
     _filled: int = 0               # chunks already copied into the ring
     _ring: object = None           # (slices, scales or None) of `_slots` chunk slots
     _slots: int = 1
-    _last_ticket: int = 0
+    _last: object = None           # future of the last tick that holds the job
     _order: int | None = None
     _slice_bytes: int | None = None
     _prep: object = None           # future of the host-side admission work
@@ -99,14 +102,20 @@ class Job:
 
 
 class _Lane:
-    """One batch stream of the server with its ticks in flight."""
+    """One batch stream of the server with its ticks in flight.  Its stream work is enqueued
+    by one worker thread, in submission order; only that thread calls into the batch while a
+    run is in progress."""
 
     def __init__(self, device, depth, wave_items):
+        import concurrent.futures
         self.batch = F.FRBatch(device)
         if wave_items:
             self.batch.set_wave_items(wave_items)
         self.depth = depth
-        self.inflight = collections.deque()   # (ticket, work) of enqueued ticks
+        self.worker = concurrent.futures.ThreadPoolExecutor(max_workers=1)
+        # (future, work) of submitted ticks; a future's result is (event recorded after the
+        # tick's launches, voxel-updates counted)
+        self.inflight = collections.deque()
 
 
 class Server:
@@ -119,7 +128,7 @@ class Server:
                  age_weight: float = 2.0, tick_seconds: float | None = 1e-2,
                  share: float | None = 0.5, min_tick_work: float = 512 * 257 ** 2,
                  fast_seconds: float | None = 2e-2, wave_items: int = 2,
-                 ring_bytes: float = 512e6):
+                 ring_bytes: float = 512e6, poll_seconds: float = 2e-4):
         import torch
         self.torch = torch
         self.dev = torch.device("cuda", device)
@@ -165,6 +174,7 @@ class Server:
         # consecutive chunks as the ring holds: a job advances by up to 8 chunks per tick
         # instead of one.  The copies are exact, so results do not change.
         self.ring_bytes = ring_bytes
+        self.poll_seconds = poll_seconds
         self._rate = None          # work units per second, measured from completed ticks
         self._main = _Lane(device, depth, wave_items)
         # the fast lane's jobs are small by construction: all their chunks are enqueued at
@@ -290,9 +300,42 @@ class Server:
                 t.record_stream(stream)
             job._filled += 1
 
+    @staticmethod
+    def _done(fut) -> bool:
+        """Has a submitted tick run on the GPU?  (Raises what the tick's thread raised.)"""
+        return fut.done() and fut.result()[0].query()
+
+    def _retire(self, lane) -> float:
+        """Drops the lane's completed ticks (in order); returns the work they held."""
+        work = 0.0
+        while lane.inflight and self._done(lane.inflight[0][0]):
+            fut, w = lane.inflight.popleft()
+            self.total_updates += fut.result()[1]
+            work += w
+        return work
+
+    def _enqueue(self, lane, plan, caller, count_updates):
+        """The lane thread's part of a tick: ring copies, the step, an event after it."""
+        torch = self.torch
+        stream = lane.batch.stream
+        with torch.cuda.stream(caller):
+            # the slices' producers (on the caller's stream) precede the copies into the rings
+            stream.wait_stream(caller)
+            bjobs = []
+            for j, _, e in plan:
+                self._fill(j, e, stream)
+                bjobs.append(F.BatchJob(j._ring[0], j.volume, j.weights, j._spaces,
+                                        self._settings(j.radius), j._table, scales=j._ring[1]))
+            lane.batch.step(bjobs, [b for _, b, _ in plan], [e for _, _, e in plan],
+                            self.strategy)
+            n = lane.batch.last_updates() if count_updates else 0
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        return ev, n
+
     def _tick(self, lane, ready, now, count_updates):
-        """Enqueues one tick of `lane` over its ready jobs (in priority order); returns the
-        work it holds."""
+        """Submits one tick of `lane` over its ready jobs (in priority order) to the lane's
+        thread; returns the work it holds."""
         # the tick: jobs in priority order, each with the spaces from its cursor to the end of
         # its ring window that fit its share of the quantum (the whole quantum when it is
         # alone), until the quantum is full; "fair", and the fast lane, give every ready job
@@ -313,23 +356,13 @@ class Server:
                 n_sp = max(1, min(left, int(min(cap, room if plan else cap) / j.space_cost())))
             plan.append((j, j._cursor, j._cursor + n_sp))
             work += n_sp * j.space_cost()
-        stream = lane.batch.stream
-        # the slices' producers (on the caller's stream) precede the copies into the rings
-        stream.wait_stream(self.torch.cuda.current_stream(self.dev))
-        bjobs = []
-        for j, _, e in plan:
-            self._fill(j, e, stream)
-            bjobs.append(F.BatchJob(j._ring[0], j.volume, j.weights, j._spaces,
-                                    self._settings(j.radius), j._table, scales=j._ring[1]))
-        lane.batch.step(bjobs, [b for _, b, _ in plan], [e for _, _, e in plan], self.strategy)
-        if count_updates:    # (the counting variant waits for each step)
-            self.total_updates += lane.batch.last_updates()
-        ticket = lane.batch.record()
-        lane.inflight.append((ticket, work))
+        fut = lane.worker.submit(self._enqueue, lane, plan,
+                                 self.torch.cuda.current_stream(self.dev), count_updates)
+        lane.inflight.append((fut, work))
         for j, _, e in plan:
             if j.started is None:
                 j.started = time.perf_counter() - self._t0
-            j._last_ticket = ticket
+            j._last = fut
             j._cursor = e
         return work
 
@@ -363,32 +396,30 @@ class Server:
                     pending[0].footprint(self.chunk, self.ring_bytes) > self.budget:
                 raise ValueError(f"job {pending[0].job_id} needs more than the HBM budget")
             # completions
-            while main.inflight and main.batch.query(main.inflight[0][0]):
-                done_work += main.inflight.popleft()[1]
+            w = self._retire(main)
+            if w:
+                done_work += w
                 if busy_since is not None and now > busy_since:
                     self._rate = done_work / (now - busy_since)
             if not main.inflight:
                 busy_since, done_work = None, 0.0
             if self._fast is not None:
-                fast = self._fast
-                while fast.inflight and fast.batch.query(fast.inflight[0][0]):
-                    fast.inflight.popleft()
+                self._retire(self._fast)
             for j in list(active):
                 if self._ready(j) and j._cursor == j.n_spaces() and \
-                        j._lane.batch.query(j._last_ticket):
+                        (j._last is None or self._done(j._last)):
                     j.done = time.perf_counter() - t0
                     active.remove(j)
                     n_main -= j._lane is main
                     self.used -= j.footprint(self.chunk, self.ring_bytes)
                     j._ring = j._spaces = None   # the volume and weights stay: the result
                     finished.append(j)
-            ready_any = ticked = False
+            ticked = False
             for lane in lanes:
                 ready = [j for j in active if j._lane is lane and j._spaces is not None and
                          j._cursor < j.n_spaces()]
                 if not ready:
                     continue
-                ready_any = True
                 if len(lane.inflight) >= lane.depth:
                     continue
                 if self.policy == "oldest":
@@ -409,12 +440,15 @@ class Server:
                     self._tick(lane, ready, now, count_updates)
             if ticked:
                 continue
-            if not ready_any and not pending:
-                time.sleep(0)
-            else:
-                time.sleep(20e-6)
+            # idle: sleep long enough for the lane threads to take the interpreter lock (a
+            # loop that takes it back at once keeps them waiting for the interpreter's switch
+            # interval, 5 ms, every time they return from a native call)
+            time.sleep(self.poll_seconds)
         for ln in lanes:
+            for fut, _ in ln.inflight:
+                fut.result()
             ln.batch.synchronize()
+            self._retire(ln)
         return finished
 
 
