@@ -1,5 +1,6 @@
 """Host-side bookkeeping of the serving loop (no GPU): a job's ring of chunk slots, its
-footprint, the window of spaces one step may cover and the fast-lane classification."""
+footprint, the window of spaces one step may cover, the fast-lane classification and the
+retirement of a lane's ticks."""
 import types
 
 import numpy as np
@@ -58,3 +59,44 @@ def test_fast_lane_classification():
     assert not S.Server._is_small(srv, job(n=8000, sym="D2"))   # 32,000 spaces: 135 ms
     srv._fast = None
     assert not S.Server._is_small(srv, small)
+
+
+class _Event:
+    def __init__(self, done):
+        self.done = done
+
+    def query(self):
+        return self.done
+
+
+def test_lane_ticks_retire_in_order_and_count_updates():
+    """A lane's ticks leave its queue in submission order once their thread has enqueued them
+    and their event has completed; their voxel-updates are added once."""
+    import concurrent.futures
+    srv = types.SimpleNamespace(total_updates=0, _done=S.Server._done)
+    lane = types.SimpleNamespace(inflight=__import__("collections").deque())
+
+    def tick(done, n):
+        f = concurrent.futures.Future()
+        if done is not None:
+            f.set_result((_Event(done), n))
+        return f
+
+    first, second, third = tick(True, 5), tick(False, 7), tick(True, 11)
+    for f, w in ((first, 1.0), (second, 2.0), (third, 4.0)):
+        lane.inflight.append((f, w))
+    # the second tick's launches have not run: the third waits behind it
+    assert S.Server._retire(srv, lane) == 1.0
+    assert srv.total_updates == 5 and len(lane.inflight) == 2
+    second.result()[0].done = True
+    assert S.Server._retire(srv, lane) == 6.0
+    assert srv.total_updates == 23 and not lane.inflight
+    # a tick whose thread has not enqueued it yet is not done
+    pending = tick(None, 0)
+    assert not S.Server._done(pending)
+    # a failure on the lane's thread is raised in the serving loop
+    bad = concurrent.futures.Future()
+    bad.set_exception(RuntimeError("umpa_fr_batch_step failed"))
+    lane.inflight.append((bad, 1.0))
+    with pytest.raises(RuntimeError, match="batch_step"):
+        S.Server._retire(srv, lane)
