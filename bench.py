@@ -145,6 +145,13 @@ def uniform_rotations(n, seed):
 KERNEL_SOURCE = os.path.join(ROOT, "umpalumpa_amd", "csrc", "fr_kernels.hip")
 
 
+# The kernel a plain Execute launches under each strategy with the library's automatic choice
+# (the operation here is not autotuned): exact -> staged ring, fast-math FP32 -> queues carried
+# across spaces, FP16 slices -> per-(brick, space) queues.
+PLAIN_KERNEL = {"exact": "fr_insert_staged_kernel", "fast": "fr_insert_carry_kernel",
+                "f16": "fr_insert_kernel"}
+
+
 def kernel_source_sha256():
     import hashlib
     try:
@@ -883,7 +890,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profile_figure("traffic_bytes_per_launch", args.strategy)
                      if (box, chunk) == (256, 1000) else None,
-                     "kernel": "fr_insert_kernel", "alg_bytes_per_launch": b_alg},
+                     "kernel": PLAIN_KERNEL[args.strategy], "alg_bytes_per_launch": b_alg},
     }
     valu = profile_figure("valu_insts_per_launch", args.strategy) \
         if (box, chunk) == (256, 1000) else None
