@@ -227,8 +227,9 @@ int umpa_fr_batch_last_updates(umpa_fr_batch *b, uint64_t *updates);
  * while a launch runs and another stream's launches (a second batch) start at once instead of
  * after it; 0 (the default) keeps one persistent workgroup per CU.  Results do not change. */
 int umpa_fr_batch_set_wave_items(umpa_fr_batch *b, uint32_t itemsPerWave);
-/* Persistent launches (itemsPerWave 0) of the batch use at most this many workgroups, one per
- * CU, leaving the other CUs to another stream's launches; 0 (the default): one per CU. */
+/* Persistent launches (itemsPerWave 0) of the batch's later steps (and plans) use at most this
+ * many workgroups, one per CU, leaving the other CUs to another stream's launches; 0 (the
+ * default): one per CU.  Results do not change. */
 int umpa_fr_batch_set_max_workgroups(umpa_fr_batch *b, uint32_t workgroups);
 
 /* records an event on the batch stream after the last enqueued work; query / wait on it */
