@@ -693,11 +693,16 @@ __device__ __forceinline__ int slab_queue(WaveScratch &ws, const FrJob &a, const
   const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
               t22 = sp.transformInv[2][2];
   const float cz = t20 * b.px + t21 * b.py;
+  // a column outside the volume compares against -1, which no square passes: the test is one
+  // compare whose lane mask is the ballot (with the column flag and-ed in, this compiler turned
+  // the mask into a 0/1 value and back, two more instructions per level: 1.227 -> 1.205 ms per
+  // box-256 chunk under fast math)
+  const float lim = b.colIn ? slab2 : -1.f;
   int cnt = 0;
   for (int k = 0; k < b.nz; ++k) {
     const float pz = (float)(b.z0 + k - cyi);
     const float qz = cz + t22 * pz;
-    const bool ok = b.colIn && !(qz * qz > slab2);
+    const bool ok = !(qz * qz > lim);
     const unsigned long long m = __ballot(ok);
     if (ok) ws.queue[cnt + (int)lanes_below(m)] = (uint16_t)(k * 64 + lane);
     cnt += __popcll(m);
@@ -918,12 +923,13 @@ __device__ __forceinline__ int slab_queue_carry(CarryScratch &ws, const FrJob &a
   const float t20 = sp.transformInv[2][0], t21 = sp.transformInv[2][1],
               t22 = sp.transformInv[2][2];
   const float cz = t20 * b.px + t21 * b.py;
+  const float lim = b.colIn ? slab2 : -1.f;  // as in slab_queue
   int cnt = 0;
   const uint32_t tag = (uint32_t)slot << 9;
   for (int k = 0; k < b.nz; ++k) {
     const float pz = (float)(b.z0 + k - cyi);
     const float qz = cz + t22 * pz;
-    const bool ok = b.colIn && !(qz * qz > slab2);
+    const bool ok = !(qz * qz > lim);
     const unsigned long long m = __ballot(ok);
     if (ok) ws.queue[pending + cnt + (int)lanes_below(m)] = (uint16_t)((uint32_t)(k * 64 + lane) | tag);
     cnt += __popcll(m);
