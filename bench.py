@@ -526,6 +526,7 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
             print(f"# serving: {k + 1} jobs served alone", file=sys.stderr, flush=True)
     print(f"# serving capacity {capacity:.2f} jobs/s ({slices / makespan:.0f} traverse spaces/s),"
           f" makespan {makespan:.2f}s", file=sys.stderr, flush=True)
+    srv.close()
     iso_arr = np.array(list(iso.values()))
     rates = []
     for policy in policies:
@@ -565,6 +566,7 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
                   f"{rates[-1]['slowdown_p99_median']:.2f} max {rates[-1]['slowdown_p99_max']:.2f}"
                   f", latency p50 {rates[-1]['latency_p50_s_median']:.3f} s", file=sys.stderr,
                   flush=True)
+        srv_p.close()
     return {"jobs": n_jobs, "chunk": chunk, "hbm_budget_gb": hbm_budget / 1e9,
             "capacity_jobs_per_s": capacity, "makespan_all_at_once_s": makespan,
             "traverse_spaces_per_s": slices / makespan,

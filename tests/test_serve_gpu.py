@@ -44,8 +44,8 @@ def make_jobs(half=False, seed=11, chunk=64):
 
 def solo(job_host, strategy, chunk=64):
     job, _ = job_host
-    srv = S.Server(chunk=chunk, strategy=strategy)
-    srv.run([job], realtime=False)
+    with S.Server(chunk=chunk, strategy=strategy) as srv:   # close() ends its threads
+        srv.run([job], realtime=False)
     return job.volume.cpu().numpy(), job.weights.cpu().numpy()
 
 

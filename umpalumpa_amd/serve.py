@@ -191,6 +191,22 @@ class Server:
         self.used = 0
         self.total_updates = 0
 
+    def close(self) -> None:
+        """Waits for the work in flight and ends the server's threads (admission workers and
+        lane threads); the server takes no further jobs."""
+        for ln in (self._main, self._fast):
+            if ln is not None:
+                ln.worker.shutdown(wait=True)
+                ln.batch.synchronize()
+        self._workers.shutdown(wait=True)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+        return False
+
     def _settings(self, radius):
         return F.FRSettings(F.FR_LOOKUP, 0, F.FR_PRECISE, self.alpha, radius)
 
