@@ -36,7 +36,10 @@ from . import fr as F
 @dataclass
 class Job:
     """A reconstruction request.  ``slices(c)`` returns the device tensor of chunk c
-    ([m, Hy, Hx] complex64, or ([m, Hy, Hx, 2] float16, [m] float32 scales) for FP16)."""
+    ([m, Hy, Hx] complex64, or ([m, Hy, Hx, 2] float16, [m] float32 scales) for FP16).  The
+    server calls it from the serving loop (chunk 0, for its shape) and from its lane threads
+    (each chunk it copies into the job's ring), there with the loop's current stream as the
+    current stream; the copies follow that stream's work."""
     job_id: int
     box: int
     n_slices: int
