@@ -72,15 +72,15 @@ def test_bounded_launches_match_oracle(items, graph):
         assert j.weights.cpu().numpy().tobytes() == rw.tobytes(), (box, items)
 
 
-@pytest.mark.parametrize("workgroups", [1, 7, 200])
-def test_capped_persistent_launches_match_oracle(workgroups):
-    """Persistent launches on at most N workgroups (umpa_fr_batch_set_max_workgroups): the same
-    bits as on every CU."""
+@pytest.mark.parametrize("workgroups,graph", [(1, False), (7, False), (200, False), (7, True)])
+def test_capped_persistent_launches_match_oracle(workgroups, graph):
+    """Persistent launches on at most N workgroups (umpa_fr_batch_set_max_workgroups), also
+    captured into the plan's HIP graph: the same bits as on every CU."""
     s = T.settings()
     host, jobs = make_jobs(SPECS, s, seed=31)
     b = F.FRBatch(0)
     b.set_max_workgroups(workgroups)
-    b.plan(jobs, 11, F.FR_EXACT, graph=False)
+    b.plan(jobs, 11, F.FR_EXACT, graph=graph)
     b.execute()
     b.synchronize()
     for (box, fft, spaces, desc, table, _), j in zip(host, jobs):
