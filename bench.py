@@ -56,6 +56,7 @@ in chunks of 512.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -509,16 +510,34 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
     def fresh():
         return serving_jobs(n_jobs, 7, chunk, pools)
 
+    def clean_start():
+        # Finished jobs keep their volumes (the results), so torch's caching allocator fills
+        # the device run after run; a failed allocation then frees every cached block, one
+        # device-wide synchronisation each, which stalls a run for seconds.  Each timed run
+        # starts from an empty cache instead (the previous run's jobs released), and the
+        # allocator's retry count is recorded per run.
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        return torch.cuda.memory_stats().get("num_alloc_retries", 0)
+
+    def retries_since(r0):
+        return torch.cuda.memory_stats().get("num_alloc_retries", 0) - r0
+
     srv = S.Server(chunk=chunk, hbm_budget=hbm_budget, strategy=strategy)
     srv.run(fresh()[:2], realtime=False)   # warm-up
     jobs = fresh()
     slices = sum(j.n_slices * j._k() for j in jobs)
+    r0 = clean_start()
     t = time.perf_counter()
     srv.run(jobs, realtime=False)
     makespan = time.perf_counter() - t
+    retries = {"all_at_once": retries_since(r0)}
+    jobs = None
     capacity = n_jobs / makespan
     print(f"# serving: {n_jobs} jobs at once in {makespan:.2f}s", file=sys.stderr, flush=True)
     iso = {}
+    r0 = clean_start()
     for k, j in enumerate(fresh()):
         srv.run([j], realtime=False)
         iso[j.job_id] = j.done - j.arrival
@@ -527,6 +546,7 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
     print(f"# serving capacity {capacity:.2f} jobs/s ({slices / makespan:.0f} traverse spaces/s),"
           f" makespan {makespan:.2f}s", file=sys.stderr, flush=True)
     srv.close()
+    retries["served_alone"] = retries_since(r0)
     iso_arr = np.array(list(iso.values()))
     rates = []
     for policy in policies:
@@ -538,12 +558,15 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
                 rng = np.random.default_rng(seed)
                 arrivals = np.cumsum(rng.exponential(1.0 / lam, size=n_jobs))
                 arrivals -= arrivals[0]
+                jobs = done = None
+                r0 = clean_start()
                 jobs = fresh()
                 for j, a in zip(jobs, arrivals):
                     j.arrival = float(a)
                 t = time.perf_counter()
                 done = srv_p.run(jobs, realtime=True)
                 span = time.perf_counter() - t
+                n_retries = retries_since(r0)
                 lat = np.array([j.done - j.arrival for j in done])
                 slow = np.array([(j.done - j.arrival) / iso[j.job_id] for j in done])
                 per_seed.append({"seed": seed, "jobs_per_s": n_jobs / span,
@@ -551,7 +574,8 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
                                  "latency_p99_s": float(np.percentile(lat, 99)),
                                  "slowdown_p50": float(np.percentile(slow, 50)),
                                  "slowdown_p99": float(np.percentile(slow, 99)),
-                                 "slowdown_max": float(slow.max())})
+                                 "slowdown_max": float(slow.max()),
+                                 "alloc_retries": n_retries})
             med = lambda key: float(np.median([r[key] for r in per_seed]))
             rates.append({"policy": policy, "fraction_of_capacity": frac,
                           "rate_jobs_per_s": lam,
@@ -573,6 +597,7 @@ def serving_summary(torch, S, F, n_jobs, fracs, policies, seeds, hbm_budget):
             "isolated_runtime_s": {"p50": float(np.percentile(iso_arr, 50)),
                                    "p99": float(np.percentile(iso_arr, 99)),
                                    "sum": float(iso_arr.sum())},
+            "alloc_retries": retries,
             "rates": rates}
 
 
@@ -906,8 +931,7 @@ def main():
         # the metric's second half, outside the timed region: the batched workload against
         # the plain operation under the same fast-math strategy, and the serving loop's
         # latency and per-job slowdown near and at saturated throughput (Poisson arrivals at
-        # 0.9 and 1.0 of the rate the server sustains with every job present; at 1.0 the
-        # queue is critically loaded and its backlog can grow through a run)
+        # 0.9 and 1.0 of the rate the server sustains with every job present)
         from umpalumpa_amd import serve as S
         summ = batched_summary(torch, F, 16, 4000, 3, 1, args.retune)
         for k in [k for k in summ if k.startswith("_")]:
