@@ -19,6 +19,12 @@ them, instead of queueing behind its ticks.  By default each tick serves the job
 least remaining work first ("srpt"), so short jobs overtake long ones; "aged" lets a long job
 that has waited take its turn.
 
+Device memory comes from torch's caching allocator.  Finished jobs keep their volumes (the
+results) until the caller drops them, and when an allocation fails the allocator frees every
+cached block, each free synchronising the device: a multi-second stall of the loop.  A
+long-running caller releases finished volumes and empties the cache at quiet moments (bench.py
+does so before each timed run).
+
 The reference has no serving layer; its harness runs one reconstruction at a time
 (examples/fourier_reconstruction/fr.cpp:9-76).
 """
