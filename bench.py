@@ -16,9 +16,11 @@ oracle's restatement where that is absent) timed on the host cores over one chun
 timed steps (the headline value does not include them) the default line adds two keys for the
 metric's second half: "batched", the batched workload below against the same jobs through
 the plain FP32 operation under the same fast-math strategy (ms per job set, GVoxel-updates/s,
-per-job latency), and "serving", 200 jobs of the serving workload below at saturated
-throughput (Poisson arrivals at the rate the server sustains with every job present): p50 /
-p99 latency and p50 / p99 per-job slowdown, a job's latency over its runtime served alone.
+per-job latency), and "serving", 200 jobs of the serving workload below near and at saturated
+throughput (Poisson arrivals at 0.9 and 1.0 of the rate the server sustains with every job
+present): p50 / p99 latency and p50 / p99 per-job slowdown, a job's latency over its runtime
+served alone.  Each timed serving run starts from an empty allocator cache and records the
+allocator's retries (a retry is a flush of every cached block in the middle of the run).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 2] [--strategy fast|exact|f16]
                     [--no-extras] [--serve-jobs 200]
@@ -85,7 +87,7 @@ def parse():
                         "workload's bar); exact: bit-identical to FRCPU; f16: FP16 slices")
     p.add_argument("--workload", default="plain",
                    choices=["plain", "batched", "serving", "large_d7", "pipeline"])
-    p.add_argument("--rates", default="0.5,0.8,1.0",
+    p.add_argument("--rates", default="0.5,0.8,0.9,1.0",
                    help="serving: arrival rates as fractions of the measured capacity")
     p.add_argument("--policies", default="srpt,aged",
                    help="serving: tick policies (srpt, aged, oldest, fair)")
