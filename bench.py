@@ -148,9 +148,10 @@ def uniform_rotations(n, seed):
 KERNEL_SOURCE = os.path.join(ROOT, "umpalumpa_amd", "csrc", "fr_kernels.hip")
 
 
-# The kernel a plain Execute launches under each strategy with the library's automatic choice
-# (the operation here is not autotuned): exact -> staged ring, fast-math FP32 -> queues carried
-# across spaces, FP16 slices -> per-(brick, space) queues.
+# The kernel a plain Execute of the box-256, 1,000-slice chunk launches under each strategy
+# with the library's automatic choice (the operation here is not autotuned): exact -> staged
+# ring, fast-math FP32 -> queues carried across spaces, FP16 slices -> per-(brick, space)
+# queues.  (Other sizes may fall back to the per-pair kernel where the others' limits bind.)
 PLAIN_KERNEL = {"exact": "fr_insert_staged_kernel", "fast": "fr_insert_carry_kernel",
                 "f16": "fr_insert_kernel"}
 
@@ -919,7 +920,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profile_figure("traffic_bytes_per_launch", args.strategy)
                      if (box, chunk) == (256, 1000) else None,
-                     "kernel": PLAIN_KERNEL[args.strategy], "alg_bytes_per_launch": b_alg},
+                     "kernel": PLAIN_KERNEL[args.strategy] if (box, chunk) == (256, 1000)
+                     else "fr_insert_* (automatic choice)", "alg_bytes_per_launch": b_alg},
     }
     valu = profile_figure("valu_insts_per_launch", args.strategy) \
         if (box, chunk) == (256, 1000) else None
